@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Benchmark of the coarse diff2 search on one MI355X.
+
+Workload: 512 particles x 4608 orientations (HEALPix order 2 x 24 psi) x 113
+translations (integer shifts, r^2 <= 36) x 1636 in-radius pixels of a 64^2
+coarse image, one 128^3 class padded 2x; synthetic, seeded inputs.  A step
+is one pass of the whole batched search over that batch (image prep,
+projections, |P|^2 norms, cross term + diff2 assembly, per-particle argmin)
+with inputs already resident in HBM.
+
+Prints one JSON line.  ``value`` is diff2 evaluations per second of the
+BF16x3 form; the FP32-MFMA form's rate is reported beside it.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "t2-relion-kernel_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp16": 2516.6}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_form(lib, torch, prob, projs, form, steps, warmup):
+    plan = lib.plan_for_problem(prob, projs, form)
+    t = lib.problem_tensors(prob)
+    plan.timing(max(steps, 1))
+    for _ in range(warmup):
+        plan.execute(t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.execute(t)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    cross_ms = plan.timing_read(steps)
+    res = dict(form=form, ms_per_step=(t1 - t0) * 1e3 / steps,
+               cross_ms=sum(cross_ms) / len(cross_ms), variant=plan.tile(),
+               plan_bytes=plan.device_bytes())
+    return res, t, plan
+
+
+def cpu_baseline(prob, target_s):
+    """float32 port of the CUDA-core loop on this host's cores, on whole
+    particles of the same workload until about target_s seconds pass."""
+    from oracle import oracle
+    pb = prob.particles
+    threads = min(16, os.cpu_count() or 1)
+    evals, t_used, n = 0, 0.0, 0
+    while n < prob.n_particles and (t_used < target_s or n == 0):
+        tx, ty = prob.trans_of(n)
+        t0 = time.perf_counter()
+        _, used = oracle.diff2_f32(prob.vols[0], prob.geom.maxR2_padded, prob.eulers, tx, ty,
+                                   pb.real[n], pb.imag[n], pb.corr[n], pb.coor_xy, pb.npix,
+                                   threads=threads)
+        t_used += time.perf_counter() - t0
+        evals += tx.size * prob.orientation_num
+        n += 1
+    return dict(value=evals / t_used, unit="diff2 evals/s", cores=used, kind="port",
+                sample=f"{n} particle(s) x {prob.orientation_num} orientations x "
+                       f"{int(prob.trans_of(0)[0].size)} translations x {pb.full_npix} pixels, "
+                       f"{t_used:.1f} s, float32 port of cuda_kernel_diff2_coarse with OpenMP")
+
+
+def fp64_check(lib, torch, prob, outs, n_orient):
+    """Error of each form against the float64 oracle on particle 0 (first
+    n_orient orientations), and argmin agreement between the forms."""
+    import numpy as np
+    from oracle import oracle
+    ref = oracle.problem_diff2(prob, 0, orient=np.arange(n_orient)).ravel()
+    T = int(prob.trans_of(0)[0].size)
+    rep = {}
+    for form, t in outs.items():
+        d = t["diff2"][: n_orient * T].double().cpu().numpy()
+        st = oracle.error_stats(d, ref)
+        rep[form] = dict(max_rel=st["max_rel"], mean_rel=st["mean_rel"])
+    forms = list(outs)
+    if len(forms) > 1:
+        a = outs[forms[0]]["best_index"].cpu().numpy()
+        for f in forms[1:]:
+            b = outs[f]["best_index"].cpu().numpy()
+            rep[f"argmin_agree_{f}_vs_{forms[0]}"] = float((a == b).mean())
+    return rep
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--particles", type=int, default=None)
+    ap.add_argument("--forms", default="bf16x3,fp32")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check-orient", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of "
+                                   "particle batches is described in DESIGN.md but not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from relion_amd import lib, synth
+
+    t0 = time.perf_counter()
+    prob = synth.make_problem(args.config, particles=args.particles)
+    log(f"generated {args.config}: {prob.n_particles} particles, O={prob.orientation_num}, "
+        f"npix={prob.particles.npix} in {time.perf_counter() - t0:.1f}s")
+    torch.cuda.set_device(0)
+    projs = [lib.Projector(v) for v in prob.vols]
+    evals = prob.evals()
+    npix = prob.particles.npix
+    forms = args.forms.split(",")
+    results, outs = {}, {}
+    for form in forms:
+        r, t, plan = time_form(lib, torch, prob, projs, form, args.steps, args.warmup)
+        r["evals_per_s"] = evals / (r["ms_per_step"] * 1e-3)
+        flops = 4.0 * npix * evals
+        r["cross_tflops"] = flops / (r["cross_ms"] * 1e-3) / 1e12
+        results[form] = r
+        outs[form] = t
+        plan.close()
+        log(f"{form}: {r['ms_per_step']:.3f} ms/step, cross {r['cross_ms']:.3f} ms, "
+            f"{r['evals_per_s']:.3e} evals/s, {r['cross_tflops']:.1f} TFLOP/s, tile {r['variant']}")
+
+    check = fp64_check(lib, torch, prob, outs, min(args.check_orient, prob.orientation_num))
+    log(f"fp64 check: {check}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(prob, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    head = forms[0]
+    r = results[head]
+    peak = PEAK_TFLOPS[head]
+    line = {
+        "metric": "diff2 evals/s @4608 orient×113 trans×1600 px; % MFMA peak; "
+                  "streamed p99 ms",
+        "value": r["evals_per_s"],
+        "unit": "diff2 evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head,
+        "data": "synthetic (seeded blob map, HEALPix x psi grid, CTF + noise particles)",
+        "config": {"workload": "coarse diff2, 512 particles x 4608 orientations x 113 "
+                               "translations x 1636 px (64^2 coarse, 128^3 map padded 2x)",
+                   "particles": prob.n_particles, "orientations": prob.orientation_num,
+                   "translations": int(prob.trans_of(0)[0].size), "npix": npix,
+                   "classes": prob.n_classes, "tile": r["variant"]},
+        "roofline": {"bound": "mfma", "achieved": r["cross_tflops"], "peak": peak,
+                     "unit": "TFLOP/s", "frac": r["cross_tflops"] / peak, "traffic": None,
+                     "kernel": f"cross_kernel<{head}>",
+                     "flops_per_launch": 4.0 * npix * evals},
+        "forms": {f: {k: v for k, v in x.items()} for f, x in results.items()},
+        "precision_vs_fp64": check,
+        "cpu_baseline": cpu,
+    }
+    if "fp32" in results and head != "fp32":
+        line["speedup_vs_fp32_mfma"] = r["evals_per_s"] / results["fp32"]["evals_per_s"]
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

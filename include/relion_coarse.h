@@ -1,0 +1,198 @@
+/*
+ * relion_coarse.h -- C ABI of the MI355X (gfx950) coarse diff2 search.
+ *
+ * Drop-in for RELION's coarse-pass squared-difference call:
+ *
+ *   diff2[o*T + t] (+)= sum_p corr_p/2 * | P_o(p) - X_p * exp(i*(x_p*tx_t + y_p*ty_t)) |^2
+ *
+ * where P_o(p) is the trilinear central-slice sample of a padded
+ * half-complex Fourier volume at the pixel rotated by orientation o.
+ *
+ * Every pointer named d_* / device below is device memory owned by the
+ * caller; host pointers are named host_*.  All calls are stream-ordered on
+ * the hipStream_t passed as `stream` (NULL = default stream), never
+ * synchronise the host, and return an rc_status; rc_last_error() gives the
+ * message of the last failure on the calling thread.  Nothing here aborts
+ * or raises a signal.
+ */
+#ifndef RELION_COARSE_H
+#define RELION_COARSE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    RC_OK = 0,
+    RC_ERR_INVALID = 1,      /* bad argument or inconsistent dimensions      */
+    RC_ERR_UNSUPPORTED = 2,  /* a mode this build does not implement          */
+    RC_ERR_NO_MEMORY = 3,    /* device allocation failed                      */
+    RC_ERR_DEVICE = 4,       /* HIP runtime / launch failure                  */
+    RC_ERR_LIMIT = 5,        /* size beyond a documented limit                */
+    RC_ERR_IO = 6            /* file missing or truncated                     */
+} rc_status;
+
+/* Arithmetic of the cross term Re(P . conj(w X e^{i phi})). */
+typedef enum {
+    RC_FORM_FP32 = 0,        /* FP32 MFMA (v_mfma_f32_16x16x4_f32), FP32 accumulate       */
+    RC_FORM_BF16X3 = 1,      /* BF16 hi/lo split, 3 MFMAs per product, FP32 accumulate    */
+    RC_FORM_FP16 = 2,        /* FP16 with per-image / per-volume power-of-two max-abs
+                                scaling, 1 MFMA per product, FP32 accumulate              */
+    RC_FORM_FP32_DIRECT = 3  /* FP32 vector ALU, direct |P - Xs|^2 form (no expansion)   */
+} rc_form;
+
+/* ---------------------------------------------------------------------- */
+/* Projector: device-resident padded Fourier volume                        */
+/* ---------------------------------------------------------------------- */
+typedef struct rc_projector rc_projector;
+
+/* Replaces AccProjector::setMdlDim + AccProjector::initMdl
+ * (acc_projector_impl.h:10-118, 120-216).  host_real/host_imag hold
+ * xdim*ydim*zdim floats, x fastest then y then z; zdim == 1 means 2D and is
+ * rejected (RC_ERR_UNSUPPORTED): only 3D references are on this path.
+ * The volume is copied to the current device with the calling thread's
+ * device; the call synchronises on that copy. */
+int rc_projector_create(int xdim, int ydim, int zdim, int inity, int initz, int maxr,
+                        float padding_factor, const float *host_real, const float *host_imag,
+                        rc_projector **out);
+
+/* Replaces AccProjector::constructFromFile (acc_projector_impl.h:300-356):
+ * int32 xdim, ydim, zdim, inity, initz, maxr; float32 padding; then the
+ * real and the imaginary planes.  Missing/short file -> RC_ERR_IO. */
+int rc_projector_create_from_file(const char *path, rc_projector **out);
+
+/* Replaces AccProjector::clear / ~AccProjector (acc_projector_impl.h:246-297). */
+int rc_projector_destroy(rc_projector *p);
+
+/* Volume dimensions of a projector (any pointer may be NULL). */
+int rc_projector_dims(const rc_projector *p, int *xdim, int *ydim, int *zdim, int *inity,
+                      int *initz, int *maxr, float *padding_factor);
+
+/* By-value view for one image geometry.  Replaces
+ * AccProjectorKernel::makeKernel (acc_projectorkernel_impl.h:410-433):
+ * maxR = min(model maxr, imgMaxR), maxR2_padded = (int)(maxR^2 * pad^2). */
+typedef struct {
+    const rc_projector *proj;
+    int imgX, imgY, imgZ;
+    int maxR, maxR2_padded;
+    float padding_factor;
+} rc_projector_kernel;
+
+int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ, int imgMaxR,
+                             rc_projector_kernel *out);
+
+/* ---------------------------------------------------------------------- */
+/* Single-image entry points (one particle, one class)                     */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces runDiff2KernelCoarse (acc_helper_functions_impl.h:15-31).
+ * Unfiltered image: image_size pixels whose (x, y) follow from the pixel
+ * index (x = i % imgX, y = i / imgX, y -= imgY if y > maxR; diff2.cuh:182-188).
+ * d_diff2s[o*T+t] is accumulated (+=).  trans_z and local_sqrtXi2 are
+ * accepted and unused, as in the reference's 2D-data path.  do_CC != 0 or
+ * data_is_3D != 0 -> RC_ERR_UNSUPPORTED.  Needs scratch device memory; it is
+ * taken from an internal per-device cache that grows on demand (the only
+ * entry point that may allocate). */
+int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
+                               const float *d_trans_x, const float *d_trans_y,
+                               const float *d_trans_z, const float *d_corr_img,
+                               const float *d_fimg_real, const float *d_fimg_imag,
+                               const float *d_eulers, float *d_diff2s, float local_sqrtXi2,
+                               unsigned long orientation_num, unsigned long translation_num,
+                               unsigned long image_size, void *stream, int do_CC,
+                               int data_is_3D);
+
+/* Replaces CoarseMatrixKernel*::get_workspace_size_bytes
+ * (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:69-88). */
+size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int image_size,
+                                 int full_image_size, int form);
+
+/* Replaces CoarseMatrixKernel*::run (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:90-131).
+ * d_real/d_imag/d_corr: the first image_size pixels are inside the
+ * projector radius; pixels image_size..full_image_size-1 (when
+ * d_coor_xy != NULL and full_image_size > image_size) only add
+ * corr/2*|X|^2.  d_coor_xy: interleaved float (x, y) per pixel, or NULL to
+ * derive coordinates from the pixel index.
+ * d_diff2s_dest[o*T+t] = (d_diff2s ? d_diff2s[o*T+t] : 0) + score; the two
+ * pointers may alias (in-place accumulation, as the reference harness uses).
+ * The workspace needs no reset between calls. */
+int rc_coarse_run(int translation_num, int orientation_num, int image_size, int full_image_size,
+                  int form, const float *d_eulers, const float *d_trans_x,
+                  const float *d_trans_y, const float *d_real, const float *d_imag,
+                  const rc_projector_kernel *projector, const float *d_corr,
+                  const float *d_diff2s, float *d_diff2s_dest, void *stream,
+                  const float *d_coor_xy, void *d_workspace, size_t workspace_bytes);
+
+/* ---------------------------------------------------------------------- */
+/* Batched entry points (no reference counterpart: the reference calls the */
+/* single-image form once per particle)                                    */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+    int n_particles;
+    int n_classes;
+    int orientation_num;        /* O, shared by all classes                      */
+    int npix;                   /* in-radius pixels (projected) per image        */
+    int full_npix;              /* total pixels per image (>= npix)              */
+    int form;                   /* rc_form (not RC_FORM_FP32_DIRECT)             */
+    const rc_projector_kernel *projectors;   /* host array [n_classes]          */
+    const int *host_trans_offset;            /* host [n_particles+1], CSR       */
+    const int *d_trans_offset;               /* device copy of the same         */
+    const float *d_trans_x, *d_trans_y;      /* device [trans_offset[P]]         */
+    const float *d_eulers;                   /* device [O*9]                     */
+    const float *d_real, *d_imag, *d_corr;   /* device [n_particles*full_npix]   */
+    const float *d_coor_xy;                  /* device [full_npix*2] (x, y)      */
+    const float *d_diff2_init;               /* device, same layout as diff2, or NULL */
+    float *d_diff2;     /* device out: particle i at n_classes*O*trans_offset[i],
+                           then [class][orientation][translation]                */
+    double *d_best_diff2;    /* device out [n_particles] (float64) or NULL        */
+    int64_t *d_best_index;   /* device out [n_particles] flat (c*O+o)*T_i+t, or NULL */
+} rc_batch_desc;
+
+typedef struct rc_plan rc_plan;
+
+/* Validate a batch shape, allocate the plan's device tables and workspace
+ * (the only allocation of the batched path) and pick tile shapes. */
+int rc_plan_create(const rc_batch_desc *desc, rc_plan **out);
+
+/* Run the batch.  Pointers in desc may differ from plan creation; the
+ * shape (particles, classes, O, npix, full_npix, form, per-particle
+ * translation counts, projector dimensions) must match.  No allocation, no
+ * host synchronisation: safe to capture in a hipGraph. */
+int rc_plan_execute(rc_plan *plan, const rc_batch_desc *desc, void *stream);
+
+int rc_plan_destroy(rc_plan *plan);
+
+/* Bytes of device memory the plan holds. */
+size_t rc_plan_device_bytes(const rc_plan *plan);
+
+/* Tile shape the plan picked: variant id and the (rows x columns) tile. */
+int rc_plan_tile(const rc_plan *plan, int *variant, int *tile_m, int *tile_n);
+
+/* Per-launch timing of the cross-term kernel: allocate n_slots pairs of
+ * hipEvents recorded on the execute stream around each cross-term launch
+ * (slot = launch count mod n_slots); n_slots = 0 turns timing off.
+ * rc_plan_timing_read waits for the slots and returns their milliseconds
+ * (up to n values, oldest first) and the number written. */
+int rc_plan_timing(rc_plan *plan, int n_slots);
+int rc_plan_timing_read(rc_plan *plan, float *ms, int n, int *n_written);
+
+/* Force the cross-term kernel variant of later rc_plan_create calls
+ * (-1 = automatic).  Used by the autotuner. */
+int rc_set_variant_override(int variant);
+
+/* Number of cross-term kernel variants and their tile shapes. */
+int rc_variant_count(int form);
+int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_gather);
+
+/* ---------------------------------------------------------------------- */
+const char *rc_last_error(void);
+const char *rc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RELION_COARSE_H */

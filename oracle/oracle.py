@@ -1,0 +1,147 @@
+"""Python side of the CPU oracle.  TEST INFRASTRUCTURE ONLY.
+
+Loads ``oracle/liboracle.so`` (built from ``coarse_oracle.c`` by the Makefile
+beside it) and wraps it for tests, the smoke check and the CPU-baseline leg
+of bench.py.  The product library never imports this module.
+
+Also holds the error metrics the reference reports (check.cuh:84-148):
+absolute error on every element, relative error only where the reference
+value exceeds 1e-12 in magnitude, mean and max of each.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-B", "-C", _HERE], check=True, capture_output=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.oracle_project_f64.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_int, _f32p, C.c_int, _f64p, _f64p,
+                                         _f64p, _f64p]
+        L.oracle_project_f64.restype = None
+        L.oracle_translate_f64.argtypes = [C.c_int, _f64p, _f64p, C.c_float, C.c_float,
+                                           _f32p, _f32p, _f64p, _f64p]
+        L.oracle_translate_f64.restype = None
+        L.oracle_diff2_f64.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_float, C.c_int, _f32p, C.c_int, _f32p, _f32p, C.c_int,
+                                       _f32p, _f32p, _f32p, _f64p, _f64p, C.c_int, C.c_int,
+                                       C.c_void_p, _f64p]
+        L.oracle_diff2_f64.restype = None
+        L.oracle_diff2_f32.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.c_float, C.c_int, _f32p, C.c_int, _f32p, _f32p, C.c_int,
+                                       _f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, _f32p, C.c_int]
+        L.oracle_diff2_f32.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _c32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _c64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _vol_args(v):
+    return (_c32(v.real), _c32(v.imag), v.mdlX, v.mdlY, v.mdlZ, v.initY, v.initZ, float(v.pad))
+
+
+def project(v, maxR2_padded, e9, xs, ys):
+    """float64 projection of one orientation at pixels (xs, ys) -> complex128."""
+    xs, ys = _c64(xs), _c64(ys)
+    re = np.zeros(xs.size)
+    im = np.zeros(xs.size)
+    lib().oracle_project_f64(*_vol_args(v), int(maxR2_padded), _c32(e9), xs.size, xs, ys, re, im)
+    return re + 1j * im
+
+
+def translate(xs, ys, tx, ty, re, im):
+    xs, ys = _c64(xs), _c64(ys)
+    o_re = np.zeros(xs.size)
+    o_im = np.zeros(xs.size)
+    lib().oracle_translate_f64(xs.size, xs, ys, float(tx), float(ty), _c32(re), _c32(im), o_re, o_im)
+    return o_re + 1j * o_im
+
+
+def diff2_f64(v, maxR2_padded, eulers, trans_x, trans_y, re, im, corr, xs, ys,
+              n_proj=None, init=None):
+    """float64 coarse diff2 of one image: returns ``[O, T]``."""
+    eulers = _c32(eulers).reshape(-1, 9)
+    O, T = eulers.shape[0], int(np.size(trans_x))
+    xs, ys = _c64(xs), _c64(ys)
+    n = xs.size
+    out = np.zeros(O * T)
+    init_p = None
+    if init is not None:
+        init = _c64(init).ravel()
+        init_p = init.ctypes.data
+    lib().oracle_diff2_f64(*_vol_args(v), int(maxR2_padded), eulers.ravel(), O, _c32(trans_x),
+                           _c32(trans_y), T, _c32(re), _c32(im), _c32(corr), xs, ys, n,
+                           n if n_proj is None else int(n_proj), init_p, out)
+    return out.reshape(O, T)
+
+
+def diff2_f32(v, maxR2_padded, eulers, trans_x, trans_y, re, im, corr, coor_xy, n_proj,
+              out=None, threads=0):
+    """float32 port of the CUDA-core loop; adds into ``out`` ``[O*T]``.
+    Returns (out, threads used)."""
+    eulers = _c32(eulers).reshape(-1, 9)
+    O, T = eulers.shape[0], int(np.size(trans_x))
+    coor = _c32(coor_xy).ravel()
+    n = coor.size // 2
+    if out is None:
+        out = np.zeros(O * T, np.float32)
+    used = lib().oracle_diff2_f32(*_vol_args(v), int(maxR2_padded), eulers.ravel(), O,
+                                  _c32(trans_x), _c32(trans_y), T, _c32(re), _c32(im),
+                                  _c32(corr), coor, n, int(n_proj), out, int(threads))
+    return out, used
+
+
+def problem_diff2(prob, i, cls=0, orient=None, init=None):
+    """float64 diff2 ``[O, T_i]`` of particle ``i`` against class ``cls`` of a
+    synthetic problem (filtered pixel order, tail = out-of-radius)."""
+    pb = prob.particles
+    tx, ty = prob.trans_of(i)
+    eul = prob.eulers if orient is None else prob.eulers[orient]
+    return diff2_f64(prob.vols[cls], prob.geom.maxR2_padded, eul, tx, ty, pb.real[i], pb.imag[i],
+                     pb.corr[i], pb.coor_xy[:, 0], pb.coor_xy[:, 1], n_proj=pb.npix, init=init)
+
+
+def error_stats(a, ref):
+    """Reference error metrics (check.cuh:96-111)."""
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(ref, np.float64).ravel()
+    ab = np.abs(a - b)
+    m = np.abs(b) > 1e-12
+    rel = ab[m] / np.abs(b[m])
+    return dict(mean_abs=float(ab.mean()) if ab.size else 0.0,
+                max_abs=float(ab.max()) if ab.size else 0.0,
+                mean_rel=float(rel.mean()) if rel.size else 0.0,
+                max_rel=float(rel.max()) if rel.size else 0.0,
+                count=int(ab.size), count_rel=int(rel.size))
+
+
+def argmin_flat(d):
+    """Lowest flat index of the minimum (ties broken by the lower index)."""
+    d = np.asarray(d).ravel()
+    return int(np.argmin(d))

@@ -1,0 +1,684 @@
+// Host runtime behind the C ABI of include/relion_coarse.h: projector
+// objects, argument validation, workspace layout, kernel selection and the
+// batched plans.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rc_internal.hpp"
+
+namespace rc {
+
+static thread_local std::string g_last_error;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+}
+
+static float pow2_scale_host(float m)
+{
+    if (!(m > 0.f)) return 1.f;
+    int e;
+    std::frexp(m, &e);
+    return std::ldexp(1.f, -e);
+}
+
+static size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Bump allocator over one device buffer.
+struct Carve {
+    unsigned char *base;
+    size_t off = 0;
+    template <class T>
+    T *take(size_t count)
+    {
+        off = align_up(off, 256);
+        T *p = base ? reinterpret_cast<T *>(base + off) : nullptr;
+        off += count * sizeof(T);
+        return p;
+    }
+};
+
+static ProjView view_of(const rc_projector_kernel &k)
+{
+    ProjView v;
+    v.vol = k.proj->vol;
+    v.mdlX = k.proj->xdim;
+    v.mdlY = k.proj->ydim;
+    v.mdlZ = k.proj->zdim;
+    v.mdlXY = k.proj->xdim * k.proj->ydim;
+    v.initY = k.proj->inity;
+    v.initZ = k.proj->initz;
+    v.maxR2_padded = k.maxR2_padded;
+    v.pad = k.padding_factor;
+    return v;
+}
+
+static int check_kernel_view(const rc_projector_kernel *k)
+{
+    RC_REQUIRE(k && k->proj && k->proj->vol, RC_ERR_INVALID, "projector kernel view is NULL");
+    RC_REQUIRE(k->imgX > 0 && k->imgY > 0, RC_ERR_INVALID, "image dimensions must be positive");
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// single-image workspace
+// ---------------------------------------------------------------------------
+struct SingleWs {
+    float2 *xw, *coords;
+    double *xnorm;
+    float *half_corr, *ascale;
+    size_t bytes;
+};
+
+static SingleWs single_layout(unsigned char *base, int npix)
+{
+    const int kpad = (int)align_up((size_t)std::max(npix, 1), kChunk);
+    Carve c{base};
+    SingleWs w;
+    w.xw = c.take<float2>(kpad);
+    w.coords = c.take<float2>(kpad);
+    w.xnorm = c.take<double>(1);
+    w.half_corr = c.take<float>(kpad);
+    w.ascale = c.take<float>(1);
+    w.bytes = align_up(c.off, 256);
+    return w;
+}
+
+static int pick_fused_variant(int form, int T)
+{
+    (void)form;
+    if (T <= 32) return 2;   // 32 x 256
+    if (T <= 64) return 1;   // 64 x 128
+    return 0;                // 128 x 128
+}
+
+static int run_single(int T, int O, int npix, int full_npix, int form, const float *eulers,
+                      const float *trans_x, const float *trans_y, const float *real,
+                      const float *imag, const rc_projector_kernel *pk, const float *corr,
+                      const float *init, float *dest, hipStream_t stream, const float *coor_xy,
+                      void *ws, size_t ws_bytes)
+{
+    if (int st = check_kernel_view(pk)) return st;
+    RC_REQUIRE(T > 0 && O > 0 && npix > 0, RC_ERR_INVALID,
+               "translation_num, orientation_num and image_size must be positive (got %d, %d, %d)",
+               T, O, npix);
+    RC_REQUIRE(full_npix >= npix, RC_ERR_INVALID, "full_image_size %d < image_size %d",
+               full_npix, npix);
+    RC_REQUIRE(npix <= kMaxPix && T <= kMaxTrans, RC_ERR_LIMIT,
+               "image_size %d or translation_num %d beyond limits (%d, %d)", npix, T, kMaxPix,
+               kMaxTrans);
+    RC_REQUIRE(form >= RC_FORM_FP32 && form <= RC_FORM_FP32_DIRECT, RC_ERR_INVALID,
+               "unknown form %d", form);
+    RC_REQUIRE(eulers && trans_x && trans_y && real && imag && corr && dest, RC_ERR_INVALID,
+               "NULL device pointer");
+    SingleWs w = single_layout((unsigned char *)ws, npix);
+    RC_REQUIRE(ws && ws_bytes >= w.bytes, RC_ERR_INVALID,
+               "workspace of %zu bytes is smaller than the %zu bytes needed", ws_bytes, w.bytes);
+    const int kpad = (int)align_up((size_t)npix, kChunk);
+    ProjView v = view_of(*pk);
+    RC_HIP(launch_prep(1, npix, full_npix, kpad, real, imag, corr, coor_xy, pk->imgX, pk->imgY,
+                       pk->maxR, w.xw, w.coords, w.xnorm, w.half_corr, w.ascale, stream));
+    if (form == RC_FORM_FP32_DIRECT) {
+        RC_REQUIRE(T <= 1024, RC_ERR_LIMIT, "direct form supports at most 1024 translations");
+        RC_HIP(launch_direct(v, eulers, O, trans_x, trans_y, T, w.coords, real, imag, corr, npix,
+                             full_npix, init, dest, stream));
+        return RC_OK;
+    }
+    const int vi = pick_fused_variant(form, T);
+    const VariantInfo *info = variant_info(form, true, vi);
+    CrossArgs a{};
+    a.groups = nullptr;
+    a.n_groups = (T + 15) / 16;
+    a.trans_offset = nullptr;
+    a.n_trans = T;
+    a.trans_x = trans_x;
+    a.trans_y = trans_y;
+    a.xw = w.xw;
+    a.coords = w.coords;
+    a.xnorm = w.xnorm;
+    a.ascale = w.ascale;
+    a.kpad = kpad;
+    a.npix = npix;
+    a.bscale = pk->proj->d_scale;
+    a.half_corr = w.half_corr;
+    a.eulers = eulers;
+    a.vol[0] = v;
+    a.n_classes = 1;
+    a.orientation_num = O;
+    a.ntiles_per_class = (O + info->tile_n - 1) / info->tile_n;
+    a.n_mtiles = (a.n_groups * 16 + info->tile_m - 1) / info->tile_m;
+    a.diff2_init = init;
+    a.diff2 = dest;
+    a.part_min = nullptr;
+    a.part_idx = nullptr;
+    RC_HIP(launch_cross(form, true, vi, a, stream));
+    return RC_OK;
+}
+
+// per-device scratch for the runDiff2KernelCoarse mirror (grows on demand)
+struct ScratchCache {
+    std::mutex mu;
+    std::vector<std::pair<void *, size_t>> per_dev;
+};
+static ScratchCache g_scratch;
+
+static int scratch_for(size_t bytes, void **out)
+{
+    int dev = 0;
+    RC_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch.mu);
+    if ((int)g_scratch.per_dev.size() <= dev) g_scratch.per_dev.resize(dev + 1, {nullptr, 0});
+    auto &e = g_scratch.per_dev[dev];
+    if (e.second < bytes) {
+        if (e.first) {
+            RC_HIP(hipDeviceSynchronize());
+            RC_HIP(hipFree(e.first));
+            e = {nullptr, 0};
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            set_error("hipMalloc of %zu scratch bytes failed", bytes);
+            return RC_ERR_NO_MEMORY;
+        }
+        e = {p, bytes};
+    }
+    *out = e.first;
+    return RC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batched plans
+// ---------------------------------------------------------------------------
+static int g_variant_override = -1;
+
+}  // namespace rc
+
+struct rc_plan {
+    int P, C, O, npix, full_npix, form, kpad, nkc;
+    int variant;
+    rc::VariantInfo info;
+    std::vector<int> trans_offset;          // host copy
+    std::vector<int> proj_dims;             // per class: 7 ints for shape checks
+    int n_groups, n_mtiles, ntiles_per_class, parts_per_group;
+    unsigned char *mem = nullptr;
+    size_t bytes = 0;
+    int2 *groups;
+    int *group_begin;
+    rc::ProjView *vols;
+    float *bscale;
+    float2 *xw, *coords;
+    double *xnorm;
+    float *half_corr, *ascale;
+    unsigned char *proj;
+    float *pabs2, *pnorm;
+    double *part_min;
+    long long *part_idx;
+    std::vector<hipEvent_t> ev;             // timing slots: 2 per launch
+    long long launches = 0;
+};
+
+namespace rc {
+
+static int default_variant(int form, int P, int Tmax)
+{
+    (void)P;
+    (void)Tmax;
+    switch (form) {
+    case kFP32: return 0;
+    case kBF16X3: return 2;
+    default: return 2;
+    }
+}
+
+static int validate_desc(const rc_batch_desc *d, bool need_ptrs)
+{
+    RC_REQUIRE(d, RC_ERR_INVALID, "NULL batch descriptor");
+    RC_REQUIRE(d->n_particles > 0 && d->n_classes > 0 && d->orientation_num > 0 && d->npix > 0,
+               RC_ERR_INVALID, "particles, classes, orientations and npix must be positive");
+    RC_REQUIRE(d->full_npix >= d->npix, RC_ERR_INVALID, "full_npix < npix");
+    RC_REQUIRE(d->npix <= kMaxPix, RC_ERR_LIMIT, "npix %d beyond %d", d->npix, kMaxPix);
+    RC_REQUIRE(d->form == RC_FORM_FP32 || d->form == RC_FORM_BF16X3 || d->form == RC_FORM_FP16,
+               RC_ERR_UNSUPPORTED, "batched path supports FP32, BF16X3 and FP16 forms (got %d)",
+               d->form);
+    RC_REQUIRE(d->projectors && d->host_trans_offset, RC_ERR_INVALID,
+               "projectors and host_trans_offset are required");
+    for (int c = 0; c < d->n_classes; ++c)
+        if (int st = check_kernel_view(&d->projectors[c])) return st;
+    RC_REQUIRE(d->host_trans_offset[0] == 0, RC_ERR_INVALID, "trans_offset[0] must be 0");
+    for (int i = 0; i < d->n_particles; ++i) {
+        const int T = d->host_trans_offset[i + 1] - d->host_trans_offset[i];
+        RC_REQUIRE(T > 0 && T <= kMaxTrans, RC_ERR_INVALID,
+                   "particle %d has %d translations (must be 1..%d)", i, T, kMaxTrans);
+    }
+    if (!need_ptrs) return RC_OK;
+    RC_REQUIRE(d->d_trans_offset && d->d_trans_x && d->d_trans_y && d->d_eulers && d->d_real &&
+                   d->d_imag && d->d_corr && d->d_coor_xy && d->d_diff2,
+               RC_ERR_INVALID, "NULL device pointer in batch descriptor");
+    return RC_OK;
+}
+
+static size_t plan_layout(rc_plan *p, unsigned char *base)
+{
+    Carve c{base};
+    const size_t nt_total = (size_t)p->C * p->ntiles_per_class;
+    p->groups = c.take<int2>(p->n_groups);
+    p->group_begin = c.take<int>(p->P + 1);
+    p->vols = c.take<ProjView>(p->C);
+    p->bscale = c.take<float>(p->C);
+    p->xw = c.take<float2>((size_t)p->P * p->kpad);
+    p->coords = c.take<float2>(p->kpad);
+    p->xnorm = c.take<double>(p->P);
+    p->half_corr = c.take<float>((size_t)p->P * p->kpad);
+    p->ascale = c.take<float>(p->P);
+    p->proj = c.take<unsigned char>((size_t)p->C * p->O * p->nkc * chunk_bytes(p->form));
+    p->pabs2 = c.take<float>((size_t)p->C * p->O * p->kpad);
+    p->pnorm = c.take<float>((size_t)p->P * p->C * p->O);
+    p->part_min = c.take<double>((size_t)p->n_groups * nt_total * p->parts_per_group);
+    p->part_idx = c.take<long long>((size_t)p->n_groups * nt_total * p->parts_per_group);
+    return align_up(c.off, 256);
+}
+
+}  // namespace rc
+
+using namespace rc;
+
+extern "C" {
+
+const char *rc_last_error(void) { return g_last_error.c_str(); }
+
+const char *rc_version(void) { return "relion_coarse_amd 0.1 (gfx950)"; }
+
+int rc_projector_create(int xdim, int ydim, int zdim, int inity, int initz, int maxr,
+                        float padding_factor, const float *host_real, const float *host_imag,
+                        rc_projector **out)
+{
+    RC_REQUIRE(out, RC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    RC_REQUIRE(zdim != 1 && zdim != 0, RC_ERR_UNSUPPORTED,
+               "2D references (zdim == 1) are not on the coarse 3D path");
+    RC_REQUIRE(xdim > 0 && ydim > 0 && zdim > 0, RC_ERR_INVALID, "bad volume dimensions");
+    RC_REQUIRE(padding_factor > 0.f && maxr > 0, RC_ERR_INVALID, "bad maxr or padding factor");
+    RC_REQUIRE(host_real && host_imag, RC_ERR_INVALID, "NULL volume data");
+    const size_t n = (size_t)xdim * ydim * zdim;
+    std::vector<float2> tmp(n);
+    float m = 0.f;
+    for (size_t i = 0; i < n; ++i) {
+        tmp[i] = make_float2(host_real[i], host_imag[i]);
+        m = std::max(m, std::max(std::fabs(host_real[i]), std::fabs(host_imag[i])));
+    }
+    rc_projector *p = new rc_projector{};
+    p->xdim = xdim;
+    p->ydim = ydim;
+    p->zdim = zdim;
+    p->inity = inity;
+    p->initz = initz;
+    p->maxr = maxr;
+    p->pad = padding_factor;
+    p->max_abs = m;
+    p->bytes = n * sizeof(float2);
+    if (hipGetDevice(&p->device) != hipSuccess ||
+        hipMalloc(&p->vol, p->bytes + 256) != hipSuccess) {
+        delete p;
+        set_error("hipMalloc of %zu bytes for the projector volume failed", n * sizeof(float2));
+        return RC_ERR_NO_MEMORY;
+    }
+    p->d_scale = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(p->vol) +
+                                           align_up(p->bytes, 256));
+    float sc = pow2_scale_host(m);
+    if (hipMemcpy(p->vol, tmp.data(), p->bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_scale, &sc, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p->vol);
+        delete p;
+        set_error("copy of the projector volume to the device failed");
+        return RC_ERR_DEVICE;
+    }
+    *out = p;
+    return RC_OK;
+}
+
+int rc_projector_create_from_file(const char *path, rc_projector **out)
+{
+    RC_REQUIRE(path && out, RC_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    std::ifstream f(path, std::ios::binary);
+    RC_REQUIRE(f.is_open(), RC_ERR_IO, "cannot open projector file '%s'", path);
+    int32_t h[6];
+    float pad = 0.f;
+    f.read(reinterpret_cast<char *>(h), sizeof(h));
+    f.read(reinterpret_cast<char *>(&pad), sizeof(pad));
+    RC_REQUIRE(f.good(), RC_ERR_IO, "projector file '%s' has a truncated header", path);
+    int zdim = h[2] == 1 ? 0 : h[2];
+    RC_REQUIRE(h[0] > 0 && h[1] > 0 && zdim > 0, RC_ERR_UNSUPPORTED,
+               "projector file '%s': dims %d x %d x %d unsupported", path, h[0], h[1], h[2]);
+    const size_t n = (size_t)h[0] * h[1] * zdim;
+    std::vector<float> re(n), im(n);
+    f.read(reinterpret_cast<char *>(re.data()), n * sizeof(float));
+    f.read(reinterpret_cast<char *>(im.data()), n * sizeof(float));
+    RC_REQUIRE(f.good(), RC_ERR_IO, "projector file '%s' is truncated", path);
+    return rc_projector_create(h[0], h[1], zdim, h[3], h[4], h[5], pad, re.data(), im.data(), out);
+}
+
+int rc_projector_destroy(rc_projector *p)
+{
+    if (!p) return RC_OK;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    if (cur != p->device) (void)hipSetDevice(p->device);
+    hipError_t e = hipFree(p->vol);
+    if (cur != p->device) (void)hipSetDevice(cur);
+    delete p;
+    RC_REQUIRE(e == hipSuccess, RC_ERR_DEVICE, "hipFree of the projector volume failed");
+    return RC_OK;
+}
+
+int rc_projector_dims(const rc_projector *p, int *xdim, int *ydim, int *zdim, int *inity,
+                      int *initz, int *maxr, float *padding_factor)
+{
+    RC_REQUIRE(p, RC_ERR_INVALID, "NULL projector");
+    if (xdim) *xdim = p->xdim;
+    if (ydim) *ydim = p->ydim;
+    if (zdim) *zdim = p->zdim;
+    if (inity) *inity = p->inity;
+    if (initz) *initz = p->initz;
+    if (maxr) *maxr = p->maxr;
+    if (padding_factor) *padding_factor = p->pad;
+    return RC_OK;
+}
+
+int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ, int imgMaxR,
+                             rc_projector_kernel *out)
+{
+    RC_REQUIRE(p && out, RC_ERR_INVALID, "NULL argument");
+    RC_REQUIRE(imgX > 0 && imgY > 0 && imgMaxR >= 0, RC_ERR_INVALID, "bad image geometry");
+    out->proj = p;
+    out->imgX = imgX;
+    out->imgY = imgY;
+    out->imgZ = imgZ;
+    out->maxR = p->maxr >= imgMaxR ? imgMaxR : p->maxr;
+    out->padding_factor = p->pad;
+    out->maxR2_padded = (int)(out->maxR * out->maxR * p->pad * p->pad);
+    return RC_OK;
+}
+
+int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float *d_trans_x,
+                               const float *d_trans_y, const float *d_trans_z,
+                               const float *d_corr_img, const float *d_fimg_real,
+                               const float *d_fimg_imag, const float *d_eulers, float *d_diff2s,
+                               float local_sqrtXi2, unsigned long orientation_num,
+                               unsigned long translation_num, unsigned long image_size,
+                               void *stream, int do_CC, int data_is_3D)
+{
+    (void)d_trans_z;
+    (void)local_sqrtXi2;
+    RC_REQUIRE(!do_CC, RC_ERR_UNSUPPORTED, "cross-correlation mode (do_CC) is not implemented");
+    RC_REQUIRE(!data_is_3D, RC_ERR_UNSUPPORTED, "3D data (data_is_3D) is not implemented");
+    if (int st = check_kernel_view(projector)) return st;
+    RC_REQUIRE(orientation_num > 0 && translation_num > 0 && image_size > 0 &&
+                   orientation_num < (1ul << 30) && translation_num <= (unsigned long)kMaxTrans &&
+                   image_size <= (unsigned long)kMaxPix,
+               RC_ERR_LIMIT, "sizes out of range");
+    RC_REQUIRE(image_size <= (unsigned long)projector->imgX * projector->imgY, RC_ERR_INVALID,
+               "image_size %lu exceeds imgX*imgY", image_size);
+    const SingleWs w = single_layout(nullptr, (int)image_size);
+    void *ws = nullptr;
+    if (int st = scratch_for(w.bytes, &ws)) return st;
+    return run_single((int)translation_num, (int)orientation_num, (int)image_size,
+                      (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real,
+                      d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
+                      (hipStream_t)stream, nullptr, ws, w.bytes);
+}
+
+size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int image_size,
+                                 int full_image_size, int form)
+{
+    (void)translation_num;
+    (void)orientation_num;
+    (void)full_image_size;
+    (void)form;
+    if (image_size <= 0) return 0;
+    return single_layout(nullptr, image_size).bytes;
+}
+
+int rc_coarse_run(int translation_num, int orientation_num, int image_size, int full_image_size,
+                  int form, const float *d_eulers, const float *d_trans_x,
+                  const float *d_trans_y, const float *d_real, const float *d_imag,
+                  const rc_projector_kernel *projector, const float *d_corr,
+                  const float *d_diff2s, float *d_diff2s_dest, void *stream,
+                  const float *d_coor_xy, void *d_workspace, size_t workspace_bytes)
+{
+    int full = full_image_size;
+    if (!d_coor_xy || full < image_size) full = image_size;
+    return run_single(translation_num, orientation_num, image_size, full, form, d_eulers,
+                      d_trans_x, d_trans_y, d_real, d_imag, projector, d_corr, d_diff2s,
+                      d_diff2s_dest, (hipStream_t)stream, d_coor_xy, d_workspace,
+                      workspace_bytes);
+}
+
+int rc_set_variant_override(int variant)
+{
+    g_variant_override = variant;
+    return RC_OK;
+}
+
+int rc_variant_count(int form) { return variant_count(form, false); }
+
+int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_gather)
+{
+    const VariantInfo *v = variant_info(form, false, variant);
+    RC_REQUIRE(v, RC_ERR_INVALID, "no variant %d for form %d", variant, form);
+    if (tile_m) *tile_m = v->tile_m;
+    if (tile_n) *tile_n = v->tile_n;
+    if (fused_gather) *fused_gather = v->fused ? 1 : 0;
+    return RC_OK;
+}
+
+int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
+{
+    RC_REQUIRE(out, RC_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (int st = validate_desc(d, false)) return st;
+    rc_plan *p = new rc_plan{};
+    p->P = d->n_particles;
+    p->C = d->n_classes;
+    p->O = d->orientation_num;
+    p->npix = d->npix;
+    p->full_npix = d->full_npix;
+    p->form = d->form;
+    p->kpad = (int)align_up((size_t)d->npix, kChunk);
+    p->nkc = p->kpad / kChunk;
+    p->trans_offset.assign(d->host_trans_offset, d->host_trans_offset + d->n_particles + 1);
+    int tmax = 0;
+    std::vector<int2> groups;
+    std::vector<int> gbeg(p->P + 1, 0);
+    for (int i = 0; i < p->P; ++i) {
+        const int T = p->trans_offset[i + 1] - p->trans_offset[i];
+        tmax = std::max(tmax, T);
+        gbeg[i] = (int)groups.size();
+        for (int t0 = 0; t0 < T; t0 += 16) groups.push_back(make_int2(i, t0));
+    }
+    gbeg[p->P] = (int)groups.size();
+    p->n_groups = (int)groups.size();
+    int vi = g_variant_override >= 0 ? g_variant_override : default_variant(p->form, p->P, tmax);
+    const VariantInfo *info = variant_info(p->form, false, vi);
+    if (!info) {
+        delete p;
+        set_error("variant %d does not exist for form %d", vi, d->form);
+        return RC_ERR_INVALID;
+    }
+    p->variant = vi;
+    p->info = *info;
+    p->n_mtiles = (p->n_groups * 16 + info->tile_m - 1) / info->tile_m;
+    p->ntiles_per_class = (p->O + info->tile_n - 1) / info->tile_n;
+    p->parts_per_group = info->waves_n;
+    for (int c = 0; c < p->C; ++c) {
+        const rc_projector *pr = d->projectors[c].proj;
+        p->proj_dims.insert(p->proj_dims.end(), {pr->xdim, pr->ydim, pr->zdim, pr->inity,
+                                                 pr->initz, d->projectors[c].maxR2_padded,
+                                                 (int)(pr->pad * 1000)});
+    }
+    p->bytes = plan_layout(p, nullptr);
+    if (hipMalloc(&p->mem, p->bytes) != hipSuccess) {
+        const size_t b = p->bytes;
+        delete p;
+        set_error("hipMalloc of %zu bytes for the batch plan failed", b);
+        return RC_ERR_NO_MEMORY;
+    }
+    plan_layout(p, p->mem);
+    std::vector<ProjView> views(p->C);
+    std::vector<float> scales(p->C);
+    for (int c = 0; c < p->C; ++c) {
+        views[c] = view_of(d->projectors[c]);
+        scales[c] = pow2_scale_host(d->projectors[c].proj->max_abs);
+    }
+    if (hipMemcpy(p->groups, groups.data(), groups.size() * sizeof(int2),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->group_begin, gbeg.data(), gbeg.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->vols, views.data(), views.size() * sizeof(ProjView),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->bscale, scales.data(), scales.size() * sizeof(float),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p->mem);
+        delete p;
+        set_error("upload of the batch plan tables failed");
+        return RC_ERR_DEVICE;
+    }
+    *out = p;
+    return RC_OK;
+}
+
+int rc_plan_execute(rc_plan *p, const rc_batch_desc *d, void *stream_)
+{
+    RC_REQUIRE(p, RC_ERR_INVALID, "NULL plan");
+    if (int st = validate_desc(d, true)) return st;
+    RC_REQUIRE(d->n_particles == p->P && d->n_classes == p->C && d->orientation_num == p->O &&
+                   d->npix == p->npix && d->full_npix == p->full_npix && d->form == p->form,
+               RC_ERR_INVALID, "batch shape differs from the plan");
+    RC_REQUIRE(std::equal(p->trans_offset.begin(), p->trans_offset.end(), d->host_trans_offset),
+               RC_ERR_INVALID, "per-particle translation counts differ from the plan");
+    for (int c = 0; c < p->C; ++c) {
+        const rc_projector *pr = d->projectors[c].proj;
+        const int *dims = &p->proj_dims[7 * c];
+        RC_REQUIRE(dims[0] == pr->xdim && dims[1] == pr->ydim && dims[2] == pr->zdim &&
+                       dims[3] == pr->inity && dims[4] == pr->initz &&
+                       dims[5] == d->projectors[c].maxR2_padded,
+                   RC_ERR_INVALID, "projector %d differs from the plan", c);
+    }
+    hipStream_t s = (hipStream_t)stream_;
+    const rc_projector_kernel &k0 = d->projectors[0];
+    RC_HIP(launch_prep(p->P, p->npix, p->full_npix, p->kpad, d->d_real, d->d_imag, d->d_corr,
+                       d->d_coor_xy, k0.imgX, k0.imgY, k0.maxR, p->xw, p->coords, p->xnorm,
+                       p->half_corr, p->ascale, s));
+    RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
+                          p->npix, p->kpad, p->proj, p->pabs2, s));
+    RC_HIP(launch_pnorm(p->P, p->C * p->O, p->kpad, p->half_corr, p->pabs2, p->pnorm, s));
+    CrossArgs a{};
+    a.groups = p->groups;
+    a.n_groups = p->n_groups;
+    a.trans_offset = d->d_trans_offset;
+    a.n_trans = 0;
+    a.trans_x = d->d_trans_x;
+    a.trans_y = d->d_trans_y;
+    a.xw = p->xw;
+    a.coords = p->coords;
+    a.xnorm = p->xnorm;
+    a.ascale = p->ascale;
+    a.kpad = p->kpad;
+    a.npix = p->npix;
+    a.proj = p->proj;
+    a.pnorm = p->pnorm;
+    a.bscale = p->bscale;
+    a.half_corr = p->half_corr;
+    a.eulers = d->d_eulers;
+    a.n_classes = p->C;
+    a.orientation_num = p->O;
+    a.ntiles_per_class = p->ntiles_per_class;
+    a.n_mtiles = p->n_mtiles;
+    a.diff2_init = d->d_diff2_init;
+    a.diff2 = d->d_diff2;
+    const bool want_best = d->d_best_diff2 || d->d_best_index;
+    a.part_min = want_best ? p->part_min : nullptr;
+    a.part_idx = want_best ? p->part_idx : nullptr;
+    const int nslot = (int)p->ev.size() / 2;
+    if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot)], s));
+    RC_HIP(launch_cross(p->form, false, p->variant, a, s));
+    if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
+    ++p->launches;
+    if (want_best)
+        RC_HIP(launch_argmin(p->P, p->group_begin,
+                             p->C * p->ntiles_per_class * p->parts_per_group, p->part_min,
+                             p->part_idx, d->d_best_diff2, (long long *)d->d_best_index, s));
+    return RC_OK;
+}
+
+static void drop_events(rc_plan *p)
+{
+    for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+    p->ev.clear();
+    p->launches = 0;
+}
+
+int rc_plan_timing(rc_plan *p, int n_slots)
+{
+    RC_REQUIRE(p && n_slots >= 0 && n_slots <= 4096, RC_ERR_INVALID, "bad timing request");
+    drop_events(p);
+    for (int i = 0; i < 2 * n_slots; ++i) {
+        hipEvent_t e;
+        RC_HIP(hipEventCreate(&e));
+        p->ev.push_back(e);
+    }
+    return RC_OK;
+}
+
+int rc_plan_timing_read(rc_plan *p, float *ms, int n, int *n_written)
+{
+    RC_REQUIRE(p && ms && n_written, RC_ERR_INVALID, "NULL argument");
+    const int nslot = (int)p->ev.size() / 2;
+    const long long have = std::min<long long>(p->launches, nslot);
+    int k = 0;
+    for (long long j = p->launches - have; j < p->launches && k < n; ++j, ++k) {
+        const int s = (int)(j % nslot);
+        RC_HIP(hipEventSynchronize(p->ev[2 * s + 1]));
+        RC_HIP(hipEventElapsedTime(&ms[k], p->ev[2 * s], p->ev[2 * s + 1]));
+    }
+    *n_written = k;
+    return RC_OK;
+}
+
+int rc_plan_destroy(rc_plan *p)
+{
+    if (!p) return RC_OK;
+    drop_events(p);
+    hipError_t e = p->mem ? hipFree(p->mem) : hipSuccess;
+    delete p;
+    RC_REQUIRE(e == hipSuccess, RC_ERR_DEVICE, "hipFree of the plan failed");
+    return RC_OK;
+}
+
+size_t rc_plan_device_bytes(const rc_plan *p) { return p ? p->bytes : 0; }
+
+int rc_plan_tile(const rc_plan *p, int *variant, int *tile_m, int *tile_n)
+{
+    RC_REQUIRE(p, RC_ERR_INVALID, "NULL plan");
+    if (variant) *variant = p->variant;
+    if (tile_m) *tile_m = p->info.tile_m;
+    if (tile_n) *tile_n = p->info.tile_n;
+    return RC_OK;
+}
+
+}  // extern "C"

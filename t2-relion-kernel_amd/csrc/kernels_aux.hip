@@ -1,0 +1,382 @@
+// Auxiliary gfx950 kernels of the coarse diff2 search: image preparation,
+// projection precompute, |P|^2 norms, per-particle argmin and the direct-form
+// float32 kernel.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "device_common.hpp"
+
+namespace rc {
+
+// ---------------------------------------------------------------------------
+// image preparation: one block per particle
+// ---------------------------------------------------------------------------
+// xw[i][p]        = corr_p * X_p            (p < npix, zero to kpad)
+// half_corr[i][p] = corr_p / 2              (p < npix, zero to kpad)
+// coords[p]       = (x_p, y_p)              (written by block 0)
+// xnorm[i]        = sum_{p < full_npix} corr_p/2 |X_p|^2   (float64)
+// ascale[i]       = power-of-two scale of max |corr X| (FP16 form)
+// Pixel coordinates come from coor_xy, or from the pixel index
+// (x = i % imgX, y = i / imgX, y -= imgY if y > maxR; diff2.cuh:182-188).
+__global__ __launch_bounds__(256) void prep_kernel(int npix, int full_npix, int kpad,
+                                                   const float *__restrict__ real,
+                                                   const float *__restrict__ imag,
+                                                   const float *__restrict__ corr,
+                                                   const float *__restrict__ coor_xy, int imgX,
+                                                   int imgY, int maxR, float2 *__restrict__ xw,
+                                                   float2 *__restrict__ coords,
+                                                   double *__restrict__ xnorm,
+                                                   float *__restrict__ half_corr,
+                                                   float *__restrict__ ascale)
+{
+    const int i = blockIdx.x;
+    const size_t src = (size_t)i * full_npix;
+    const size_t dst = (size_t)i * kpad;
+    double acc = 0.0;
+    float amax = 0.f;
+    for (int p = threadIdx.x; p < kpad; p += blockDim.x) {
+        float2 w = make_float2(0.f, 0.f);
+        float hc = 0.f;
+        if (p < npix) {
+            const float xr = real[src + p], xi = imag[src + p], c = corr[src + p];
+            w = make_float2(c * xr, c * xi);
+            hc = c * 0.5f;
+            acc += (double)hc * ((double)xr * xr + (double)xi * xi);
+            amax = fmaxf(amax, fmaxf(fabsf(w.x), fabsf(w.y)));
+        }
+        xw[dst + p] = w;
+        if (half_corr) half_corr[dst + p] = hc;
+        if (blockIdx.x == 0) {
+            float2 cxy = make_float2(0.f, 0.f);
+            if (p < npix) {
+                if (coor_xy) {
+                    cxy = make_float2(coor_xy[2 * p], coor_xy[2 * p + 1]);
+                } else {
+                    int x = p % imgX, y = p / imgX;
+                    if (y > maxR) y -= imgY;
+                    cxy = make_float2((float)x, (float)y);
+                }
+            }
+            coords[p] = cxy;
+        }
+    }
+    for (int p = npix + threadIdx.x; p < full_npix; p += blockDim.x) {
+        const float xr = real[src + p], xi = imag[src + p], c = corr[src + p];
+        acc += (double)(c * 0.5f) * ((double)xr * xr + (double)xi * xi);
+    }
+    __shared__ double s_acc[256];
+    __shared__ float s_max[256];
+    s_acc[threadIdx.x] = acc;
+    s_max[threadIdx.x] = amax;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s_acc[threadIdx.x] += s_acc[threadIdx.x + w];
+            s_max[threadIdx.x] = fmaxf(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        xnorm[i] = s_acc[0];
+        if (ascale) ascale[i] = pow2_scale(s_max[0]);
+    }
+}
+
+hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const float *real,
+                       const float *imag, const float *corr, const float *coor_xy, int imgX,
+                       int imgY, int maxR, float2 *xw, float2 *coords, double *xnorm,
+                       float *half_corr, float *ascale, hipStream_t s)
+{
+    hipLaunchKernelGGL(prep_kernel, dim3(n_particles), dim3(256), 0, s, npix, full_npix, kpad,
+                       real, imag, corr, coor_xy, imgX, imgY, maxR, xw, coords, xnorm, half_corr,
+                       ascale);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// projection precompute: block = 8 orientations x one 32-pixel chunk
+// ---------------------------------------------------------------------------
+// proj[(c*O + o)*nkc + kc] = encoded 32 samples; pabs2[(c*O+o)*kpad + p] = |P|^2
+template <int FORM>
+__global__ __launch_bounds__(256) void project_kernel(const ProjView *__restrict__ vols,
+                                                      const float *__restrict__ bscale,
+                                                      const float *__restrict__ eulers, int O,
+                                                      const float2 *__restrict__ coords, int npix,
+                                                      int kpad, unsigned char *__restrict__ proj,
+                                                      float *__restrict__ pabs2)
+{
+    constexpr int CB = chunk_bytes(FORM);
+    constexpr int GB = group_bytes(FORM);
+    __shared__ __attribute__((aligned(16))) unsigned char s_out[8 * CB];
+    const int nkc = kpad / kChunk;
+    const int kc = blockIdx.x;
+    const int cls = blockIdx.z;
+    const int ol = threadIdx.x >> 5;
+    const int pl = threadIdx.x & 31;
+    const int o = blockIdx.y * 8 + ol;
+    const int p = kc * kChunk + pl;
+    const ProjView v = vols[cls];
+    float2 val = make_float2(0.f, 0.f);
+    if (o < O && p < npix) {
+        const float *e = eulers + 9 * (size_t)o;
+        const float2 xy = coords[p];
+        val = project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+    }
+    const float sc = (FORM == kFP16) ? bscale[cls] : 1.f;
+    store_encoded<FORM>(s_out + ol * CB + (pl >> 3) * GB, pl & 7, val.x, val.y, sc);
+    if (o < O) pabs2[((size_t)cls * O + o) * kpad + p] = val.x * val.x + val.y * val.y;
+    __syncthreads();
+    constexpr int units = 8 * CB / 16;
+    for (int u = threadIdx.x; u < units; u += 256) {
+        const int oo = blockIdx.y * 8 + u / (CB / 16);
+        if (oo < O) {
+            const int seg = u % (CB / 16);
+            uint4 d = reinterpret_cast<const uint4 *>(s_out)[u];
+            reinterpret_cast<uint4 *>(proj + (((size_t)cls * O + oo) * nkc + kc) * CB)[seg] = d;
+        }
+    }
+}
+
+hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,
+                          const float *eulers, int O, const float2 *coords, int npix, int kpad,
+                          unsigned char *proj, float *pabs2, hipStream_t s)
+{
+    dim3 grid(kpad / kChunk, (O + 7) / 8, n_classes);
+    switch (form) {
+    case kFP32:
+        hipLaunchKernelGGL(project_kernel<kFP32>, grid, dim3(256), 0, s, vols, bscale, eulers, O,
+                           coords, npix, kpad, proj, pabs2);
+        break;
+    case kBF16X3:
+        hipLaunchKernelGGL(project_kernel<kBF16X3>, grid, dim3(256), 0, s, vols, bscale, eulers,
+                           O, coords, npix, kpad, proj, pabs2);
+        break;
+    default:
+        hipLaunchKernelGGL(project_kernel<kFP16>, grid, dim3(256), 0, s, vols, bscale, eulers, O,
+                           coords, npix, kpad, proj, pabs2);
+        break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// |P|^2 norms: pnorm[i][n] = sum_p half_corr[i][p] * pabs2[n][p]
+// block tile 32 particles x 64 columns, 256 threads, each 2 x 4 outputs
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pnorm_kernel(int P, int N, int kpad,
+                                                    const float *__restrict__ hc,
+                                                    const float *__restrict__ pabs2,
+                                                    float *__restrict__ pnorm)
+{
+    __shared__ float sA[32][33];
+    __shared__ float sB[64][33];
+    const int i0 = blockIdx.y * 32, n0 = blockIdx.x * 64;
+    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;   // 16 x 16
+    float acc[2][4] = {};
+    for (int k0 = 0; k0 < kpad; k0 += 32) {
+        for (int u = threadIdx.x; u < 32 * 32; u += 256) {
+            const int r = u / 32, k = u % 32;
+            sA[r][k] = (i0 + r < P) ? hc[(size_t)(i0 + r) * kpad + k0 + k] : 0.f;
+        }
+        for (int u = threadIdx.x; u < 64 * 32; u += 256) {
+            const int r = u / 32, k = u % 32;
+            sB[r][k] = (n0 + r < N) ? pabs2[(size_t)(n0 + r) * kpad + k0 + k] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int k = 0; k < 32; ++k) {
+            const float a0 = sA[ty][k], a1 = sA[ty + 16][k];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float b = sB[tx + 16 * j][k];
+                acc[0][j] = fmaf(a0, b, acc[0][j]);
+                acc[1][j] = fmaf(a1, b, acc[1][j]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int i = i0 + ty + 16 * r, n = n0 + tx + 16 * j;
+            if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[r][j];
+        }
+}
+
+hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,
+                        const float *pabs2, float *pnorm, hipStream_t s)
+{
+    dim3 grid((n_cols + 63) / 64, (n_particles + 31) / 32);
+    hipLaunchKernelGGL(pnorm_kernel, grid, dim3(256), 0, s, n_particles, n_cols, kpad, half_corr,
+                       pabs2, pnorm);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// per-particle min / argmin over the cross kernel's partials (float64)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void argmin_kernel(const int *__restrict__ group_begin,
+                                                     int parts_per_group,
+                                                     const double *__restrict__ part_min,
+                                                     const long long *__restrict__ part_idx,
+                                                     double *__restrict__ best,
+                                                     long long *__restrict__ best_idx)
+{
+    const int i = blockIdx.x;
+    const size_t a = (size_t)group_begin[i] * parts_per_group;
+    const size_t b = (size_t)group_begin[i + 1] * parts_per_group;
+    double m = DBL_MAX;
+    long long id = 0x7fffffffffffffffLL;
+    for (size_t u = a + threadIdx.x; u < b; u += blockDim.x) {
+        const double v = part_min[u];
+        const long long k = part_idx[u];
+        if (v < m || (v == m && k < id)) {
+            m = v;
+            id = k;
+        }
+    }
+    __shared__ double sm[256];
+    __shared__ long long si[256];
+    sm[threadIdx.x] = m;
+    si[threadIdx.x] = id;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            const double v = sm[threadIdx.x + w];
+            const long long k = si[threadIdx.x + w];
+            if (v < sm[threadIdx.x] || (v == sm[threadIdx.x] && k < si[threadIdx.x])) {
+                sm[threadIdx.x] = v;
+                si[threadIdx.x] = k;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (best) best[i] = sm[0];
+        if (best_idx) best_idx[i] = si[0];
+    }
+}
+
+hipError_t launch_argmin(int n_particles, const int *group_begin, int parts_per_group,
+                         const double *part_min, const long long *part_idx, double *best,
+                         long long *best_idx, hipStream_t s)
+{
+    hipLaunchKernelGGL(argmin_kernel, dim3(n_particles), dim3(256), 0, s, group_begin,
+                       parts_per_group, part_min, part_idx, best, best_idx);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// direct form, float32 vector ALU
+// ---------------------------------------------------------------------------
+// The arithmetic of cuda_kernel_diff2_coarse (diff2.cuh:369-537): per
+// (orientation, translation) accumulate corr/2 * |P - Xs|^2 over all pixels,
+// with one sincos per (translation, pixel) shared by a block's orientations.
+// Block = 16 orientations x 64 pixel lanes... laid out as 256 threads:
+// thread t handles translations t, t+256, ...; projections of the block's
+// 16 orientations for a 64-pixel chunk are staged in LDS.  Deterministic
+// (no atomics): one block owns its 16 orientations for all translations.
+constexpr int kDirOrient = 16;
+constexpr int kDirPix = 64;
+constexpr int kDirMaxTPerThread = 4;
+
+__global__ __launch_bounds__(256) void direct_kernel(ProjView v, const float *__restrict__ eulers,
+                                                     int O, const float *__restrict__ trans_x,
+                                                     const float *__restrict__ trans_y, int T,
+                                                     const float2 *__restrict__ coords,
+                                                     const float *__restrict__ real,
+                                                     const float *__restrict__ imag,
+                                                     const float *__restrict__ corr, int npix,
+                                                     int full_npix, const float *init,
+                                                     float *out)
+{
+    __shared__ float2 s_p[kDirPix][kDirOrient];
+    __shared__ float4 s_x[kDirPix];   // (xr, xi, corr/2, unused)
+    __shared__ float2 s_c[kDirPix];
+    __shared__ float s_e[kDirOrient][6];
+    const int o0 = blockIdx.x * kDirOrient;
+    if (threadIdx.x < kDirOrient * 6) {
+        const int j = threadIdx.x / 6, k = threadIdx.x % 6;
+        const int src[6] = {0, 1, 3, 4, 6, 7};
+        s_e[j][k] = (o0 + j < O) ? eulers[9 * (size_t)(o0 + j) + src[k]] : 0.f;
+    }
+    float acc[kDirMaxTPerThread][kDirOrient];
+    float txv[kDirMaxTPerThread], tyv[kDirMaxTPerThread];
+#pragma unroll
+    for (int u = 0; u < kDirMaxTPerThread; ++u) {
+        const int t = threadIdx.x + 256 * u;
+        txv[u] = t < T ? trans_x[t] : 0.f;
+        tyv[u] = t < T ? trans_y[t] : 0.f;
+#pragma unroll
+        for (int j = 0; j < kDirOrient; ++j) acc[u][j] = 0.f;
+    }
+    for (int p0 = 0; p0 < full_npix; p0 += kDirPix) {
+        __syncthreads();
+        for (int u = threadIdx.x; u < kDirPix * kDirOrient; u += 256) {
+            const int pl = u / kDirOrient, j = u % kDirOrient;
+            const int p = p0 + pl;
+            float2 val = make_float2(0.f, 0.f);
+            if (p < npix && o0 + j < O) {
+                const float2 xy = coords[p];
+                val = project_sample(v, s_e[j][0], s_e[j][1], s_e[j][2], s_e[j][3], s_e[j][4],
+                                     s_e[j][5], xy.x, xy.y);
+            }
+            s_p[pl][j] = val;
+        }
+        if (threadIdx.x < kDirPix) {
+            const int p = p0 + threadIdx.x;
+            s_x[threadIdx.x] = p < full_npix
+                                   ? make_float4(real[p], imag[p], corr[p] * 0.5f, 0.f)
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+            s_c[threadIdx.x] = p < npix ? coords[p] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        const int np = min(kDirPix, full_npix - p0);
+#pragma unroll
+        for (int u = 0; u < kDirMaxTPerThread; ++u) {
+            if (threadIdx.x + 256 * u >= T) break;
+            for (int pl = 0; pl < np; ++pl) {
+                const float4 xv = s_x[pl];
+                const float2 xy = s_c[pl];
+                float s, c;
+                sincosf(xy.x * txv[u] + xy.y * tyv[u], &s, &c);
+                const float xr = c * xv.x - s * xv.y;
+                const float xi = c * xv.y + s * xv.x;
+#pragma unroll
+                for (int j = 0; j < kDirOrient; ++j) {
+                    const float2 pv = s_p[pl][j];
+                    const float dr = pv.x - xr, di = pv.y - xi;
+                    acc[u][j] = fmaf(dr * dr + di * di, xv.z, acc[u][j]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kDirMaxTPerThread; ++u) {
+        const int t = threadIdx.x + 256 * u;
+        if (t >= T) break;
+#pragma unroll
+        for (int j = 0; j < kDirOrient; ++j) {
+            if (o0 + j < O) {
+                const size_t k = (size_t)(o0 + j) * T + t;
+                out[k] = (init ? init[k] : 0.f) + acc[u][j];
+            }
+        }
+    }
+}
+
+hipError_t launch_direct(const ProjView &v, const float *eulers, int O, const float *trans_x,
+                         const float *trans_y, int T, const float2 *coords, const float *real,
+                         const float *imag, const float *corr, int npix, int full_npix,
+                         const float *init, float *out, hipStream_t s)
+{
+    if (T > 256 * kDirMaxTPerThread) return hipErrorInvalidValue;
+    dim3 grid((O + kDirOrient - 1) / kDirOrient);
+    hipLaunchKernelGGL(direct_kernel, grid, dim3(256), 0, s, v, eulers, O, trans_x, trans_y, T,
+                       coords, real, imag, corr, npix, full_npix, init, out);
+    return hipGetLastError();
+}
+
+}  // namespace rc

@@ -1,0 +1,148 @@
+// Internal declarations shared by the host runtime and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "relion_coarse.h"
+
+namespace rc {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define RC_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            rc::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                          __LINE__);                                                      \
+            return RC_ERR_DEVICE;                                                         \
+        }                                                                                 \
+    } while (0)
+
+#define RC_REQUIRE(cond, code, ...)      \
+    do {                                 \
+        if (!(cond)) {                   \
+            rc::set_error(__VA_ARGS__);  \
+            return (code);               \
+        }                                \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// device-side views
+// ---------------------------------------------------------------------------
+// Padded Fourier volume: one float2 (re, im) per voxel, x fastest, then y,
+// then z -- the planar pair of AccProjector interleaved so one 8-byte load
+// fetches both parts of a voxel.
+struct ProjView {
+    const float2 *vol;
+    int mdlX, mdlY, mdlZ, mdlXY;
+    int initY, initZ;
+    int maxR2_padded;
+    float pad;
+};
+
+// Limits of this build (checked on the host before any launch).
+constexpr int kMaxPix = 1 << 20;          // pixels per image
+constexpr int kMaxTrans = 1 << 16;        // translations per particle
+constexpr int kChunk = 32;                // pixels per MFMA k-chunk
+constexpr int kGroupPix = 8;              // pixels per lane per chunk
+
+// Cross-term arithmetic (values match rc_form).
+enum Form : int { kFP32 = 0, kBF16X3 = 1, kFP16 = 2 };
+
+// Bytes of one encoded 8-pixel group of a projection (or of a translated
+// image row): FP32 re[8] im[8]; BF16X3 re_hi[8] re_lo[8] im_hi[8] im_lo[8];
+// FP16 re[8] im[8].
+constexpr int group_bytes(int form) { return form == kFP16 ? 32 : 64; }
+constexpr int chunk_bytes(int form) { return 4 * group_bytes(form); }
+
+// Arguments of the cross-term kernel (one struct, passed by value).
+struct CrossArgs {
+    // rows: row group g covers translations t0 .. t0+15 of one particle
+    const int2 *groups;          // [n_groups] (particle, t0); NULL => particle 0, t0 = 16 g
+    int n_groups;
+    const int *trans_offset;     // [P+1]; NULL => single particle with T = n_trans
+    int n_trans;
+    const float *trans_x, *trans_y;
+    // A side
+    const float2 *xw;            // [P][kpad] corr * X (complex), zero padded
+    const float2 *coords;        // [kpad] (x, y) pixel coordinates
+    const double *xnorm;         // [P] sum corr/2 |X|^2 over all pixels (float64)
+    const float *ascale;         // [P] FP16 form: power-of-two image scale
+    int kpad;                    // pixels padded to a multiple of kChunk
+    int npix;                    // projected pixels (the rest of kpad is zero)
+    // B side
+    const unsigned char *proj;   // [C][O][kpad/32][chunk_bytes] encoded projections
+    const float *pnorm;          // [P][C][O] sum corr/2 |P|^2
+    const float *bscale;         // [C] FP16 form: power-of-two volume scale
+    const float *half_corr;      // fused gather: [P][kpad] corr/2 (zero padded)
+    const float *eulers;         // fused gather: [O][9]
+    ProjView vol[1];             // fused gather: the (single) class volume
+    int n_classes, orientation_num;
+    int ntiles_per_class;        // ceil(O / tile_n)
+    int n_mtiles;
+    // outputs
+    const float *diff2_init;     // same layout as diff2, may be NULL, may alias diff2
+    float *diff2;
+    double *part_min;            // [n_groups][C*ntiles_per_class][waves_n] or NULL
+    long long *part_idx;
+};
+
+// Description of one compiled cross-term variant.
+struct VariantInfo {
+    int form;
+    int tile_m, tile_n;
+    int threads;
+    int waves_n;
+    bool fused;
+    int lds_bytes;
+};
+
+int variant_count(int form, bool fused);
+const VariantInfo *variant_info(int form, bool fused, int idx);
+hipError_t launch_cross(int form, bool fused, int idx, const CrossArgs &a, hipStream_t s);
+
+// Prep: corr*X, coordinates, |X|^2 (float64), FP16 image scale.
+hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const float *real,
+                       const float *imag, const float *corr, const float *coor_xy, int imgX,
+                       int imgY, int maxR, float2 *xw, float2 *coords, double *xnorm,
+                       float *half_corr, float *ascale, hipStream_t s);
+
+// Projections of every (class, orientation, pixel) into the encoded layout.
+hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,
+                          const float *eulers, int O, const float2 *coords, int npix, int kpad,
+                          unsigned char *proj, float *pabs2, hipStream_t s);
+
+// pnorm[i][c*O+o] = sum_p half_corr[i][p] * pabs2[c*O+o][p]
+hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,
+                        const float *pabs2, float *pnorm, hipStream_t s);
+
+// Per-particle min / argmin from the cross kernel's partials.
+hipError_t launch_argmin(int n_particles, const int *group_begin, int parts_per_group,
+                         const double *part_min, const long long *part_idx, double *best,
+                         long long *best_idx, hipStream_t s);
+
+// Direct-form float32 vector kernel (the original kernel's arithmetic).
+hipError_t launch_direct(const ProjView &v, const float *eulers, int O, const float *trans_x,
+                         const float *trans_y, int T, const float2 *coords, const float *real,
+                         const float *imag, const float *corr, int npix, int full_npix,
+                         const float *init, float *out, hipStream_t s);
+
+}  // namespace rc
+
+// opaque handle of the C ABI
+struct rc_projector {
+    float2 *vol;        // device, interleaved complex
+    int xdim, ydim, zdim, inity, initz, maxr;
+    float pad;
+    float max_abs;      // max |re|, |im| over the volume
+    float *d_scale;     // device float: power-of-two scale of max_abs (FP16 form)
+    int device;
+    size_t bytes;
+};

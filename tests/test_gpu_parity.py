@@ -1,0 +1,194 @@
+"""Parity of the HIP paths against the float64 CPU oracle (GPU required).
+
+Every test calls the gfx950 kernels through the C ABI (ctypes) and compares
+diff2 element-wise with ``oracle.diff2_f64`` using the reference's error
+metrics (check.cuh:96-111).  Tolerances (relative to the float64 value):
+
+* FP32 MFMA / FP32 direct forms: max 2e-5, mean 1e-6 (float32 projection,
+  phase and accumulation against float64; the reference's own float32
+  kernels sit at a mean of ~1e-7 against its float64 kernel);
+* BF16x3: max 1e-5, mean 1e-6, identical per-particle argmin;
+* FP16 (per-image scaled, one product): max 5e-3, mean 5e-4.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": (2e-5, 1e-6), "fp32_direct": (2e-5, 1e-6), "bf16x3": (1e-5, 1e-6),
+       "fp16": (5e-3, 5e-4)}
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from relion_amd import synth
+    prob = synth.make_problem("c1", seed=11)
+    ref = [oracle.problem_diff2(prob, i) for i in range(prob.n_particles)]
+    return prob, ref
+
+
+@pytest.fixture(scope="module")
+def lib(torch_cuda):
+    from relion_amd import lib as L
+    L.load()
+    return L
+
+
+def _t(torch, a, dt=None):
+    x = torch.as_tensor(np.ascontiguousarray(a))
+    if dt is not None:
+        x = x.to(dt)
+    return x.cuda()
+
+
+def _assert_close(got, ref, form, what):
+    st = oracle.error_stats(got, ref)
+    mx, mean = TOL[form]
+    print(f"{what} [{form}] max_rel={st['max_rel']:.3e} mean_rel={st['mean_rel']:.3e}")
+    assert st["max_rel"] <= mx and st["mean_rel"] <= mean, (what, form, st)
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16", "fp32_direct"])
+def test_single_run_filtered(c1, lib, torch_cuda, form):
+    """rc_coarse_run on filtered inputs (in-radius first, explicit coords),
+    diff2s_dest = diff2s + score with distinct source and destination."""
+    torch = torch_cuda
+    prob, ref = c1
+    pb = prob.particles
+    g = prob.geom
+    proj = lib.Projector(prob.vols[0])
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    O = prob.orientation_num
+    eul = _t(torch, prob.eulers.ravel())
+    coor = _t(torch, pb.coor_xy.ravel())
+    rng = np.random.default_rng(5)
+    for i in range(prob.n_particles):
+        tx, ty = prob.trans_of(i)
+        T = tx.size
+        init = rng.uniform(0, 10, O * T).astype(np.float32)
+        k = lib.CoarseKernel(T, O, pb.npix, pb.full_npix, form)
+        src = _t(torch, init)
+        dst = torch.full((O * T,), -1.0, dtype=torch.float32, device="cuda")
+        k.run(eul, _t(torch, tx), _t(torch, ty), _t(torch, pb.real[i]), _t(torch, pb.imag[i]), pk,
+              _t(torch, pb.corr[i]), src, dst, coor_xy=coor)
+        torch.cuda.synchronize()
+        got = dst.cpu().numpy().astype(np.float64)
+        _assert_close(got, ref[i].ravel() + init, form, f"particle {i}")
+        assert np.array_equal(src.cpu().numpy(), init)
+
+
+def test_run_diff2_kernel_coarse_unfiltered(c1, lib, torch_cuda):
+    """Mirror of runDiff2KernelCoarse: unfiltered image, coordinates from the
+    pixel index, diff2s accumulated in place."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob, ref = c1
+    pb = prob.particles
+    g = prob.geom
+    order, _ = synth.filter_order(g)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    proj = lib.Projector(prob.vols[0])
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    O = prob.orientation_num
+    eul = _t(torch, prob.eulers.ravel())
+    for i in (0, 3):
+        tx, ty = prob.trans_of(i)
+        T = tx.size
+        init = np.full(O * T, 2.5, np.float32)
+        d2 = _t(torch, init)
+        lib.run_diff2_kernel_coarse(pk, _t(torch, tx), _t(torch, ty), _t(torch, pb.corr[i][inv]),
+                                    _t(torch, pb.real[i][inv]), _t(torch, pb.imag[i][inv]), eul,
+                                    d2, O, T, g.image_size)
+        torch.cuda.synchronize()
+        _assert_close(d2.cpu().numpy(), ref[i].ravel() + 2.5, "fp32", f"unfiltered {i}")
+
+
+def test_run_diff2_kernel_coarse_rejects_cc(c1, lib, torch_cuda):
+    torch = torch_cuda
+    prob, _ = c1
+    g = prob.geom
+    proj = lib.Projector(prob.vols[0])
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    x = torch.zeros(4096, device="cuda")
+    with pytest.raises(lib.RcError) as e:
+        lib.run_diff2_kernel_coarse(pk, x, x, x, x, x, x, x, 1, 1, 16, do_CC=True)
+    assert e.value.status == 2
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
+def test_batched_c1(c1, lib, torch_cuda, form):
+    torch = torch_cuda
+    prob, ref = c1
+    projs = [lib.Projector(v) for v in prob.vols]
+    plan = lib.plan_for_problem(prob, projs, form)
+    t = lib.problem_tensors(prob)
+    plan.execute(t)
+    torch.cuda.synchronize()
+    d = t["diff2"].cpu().numpy().astype(np.float64)
+    off = lib.output_offsets(prob.trans_offset, prob.n_classes, prob.orientation_num)
+    best = t["best_diff2"].cpu().numpy()
+    bidx = t["best_index"].cpu().numpy()
+    for i in range(prob.n_particles):
+        got = d[off[i]:off[i + 1]]
+        _assert_close(got, ref[i].ravel(), form, f"batched particle {i}")
+        k = oracle.argmin_flat(ref[i])
+        if form != "fp16":
+            assert bidx[i] == k, (i, bidx[i], k)
+        assert abs(best[i] - got[bidx[i]]) <= 1e-6 * abs(best[i])
+        assert got[bidx[i]] == got.min()
+
+
+def test_batched_ragged_multiclass(lib, torch_cuda):
+    """Two classes, uneven translation counts (3..37 shifts), 5 particles."""
+    from relion_amd import synth
+    torch = torch_cuda
+    vols = [synth.fourier_volume(synth.blob_map(32, seed=s)) for s in (3, 4)]
+    g = synth.CoarseGeometry(imgY=16, maxR=8)
+    eul = synth.orientation_set(0, 8)
+    pb = synth.make_particles(vols, g, 5, 32, 0.5, seed=9, shift_range=1.0)
+    txs, tys, offs = [], [], [0]
+    for i, r2 in enumerate((1, 13, 4, 2, 0)):
+        tx, ty, _ = synth.translation_set(r2, 32, centre=(0.5 * i, -0.25 * i))
+        txs.append(tx)
+        tys.append(ty)
+        offs.append(offs[-1] + tx.size)
+    prob = synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
+                               np.concatenate(txs), np.concatenate(tys), pb, 32)
+    off = lib.output_offsets(prob.trans_offset, 2, prob.orientation_num)
+    for form in ("fp32", "bf16x3"):
+        projs = [lib.Projector(v) for v in vols]
+        plan = lib.plan_for_problem(prob, projs, form)
+        t = lib.problem_tensors(prob)
+        plan.execute(t)
+        torch.cuda.synchronize()
+        d = t["diff2"].cpu().numpy().astype(np.float64)
+        bidx = t["best_index"].cpu().numpy()
+        for i in range(prob.n_particles):
+            ref = np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(2)])
+            _assert_close(d[off[i]:off[i + 1]], ref.ravel(), form, f"ragged {i}")
+            assert bidx[i] == oracle.argmin_flat(ref), (form, i)
+
+
+def test_batched_variants_agree(c1, lib, torch_cuda):
+    """Every compiled tile shape gives the same FP32 result within tolerance."""
+    torch = torch_cuda
+    prob, ref = c1
+    projs = [lib.Projector(v) for v in prob.vols]
+    off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
+    try:
+        for form in ("fp32", "bf16x3"):
+            for v, tm, tn in lib.variants(form):
+                lib.set_variant_override(v)
+                plan = lib.plan_for_problem(prob, projs, form)
+                t = lib.problem_tensors(prob)
+                plan.execute(t)
+                torch.cuda.synchronize()
+                d = t["diff2"].cpu().numpy().astype(np.float64)
+                for i in (0, 7):
+                    _assert_close(d[off[i]:off[i + 1]], ref[i].ravel(), form,
+                                  f"variant {v} ({tm}x{tn}) particle {i}")
+    finally:
+        lib.set_variant_override(-1)
