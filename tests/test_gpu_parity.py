@@ -7,7 +7,7 @@ metrics (check.cuh:96-111).  Tolerances (relative to the float64 value):
 * FP32 MFMA / FP32 direct forms: max 2e-5, mean 1e-6 (float32 projection,
   phase and accumulation against float64; the reference's own float32
   kernels sit at a mean of ~1e-7 against its float64 kernel);
-* BF16x3: max 1e-5, mean 1e-6, identical per-particle argmin;
+* BF16x3: max 1e-5, mean 2e-6, identical per-particle argmin;
 * FP16 (per-image scaled, one product): max 5e-3, mean 5e-4.
 """
 import numpy as np
@@ -17,7 +17,7 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": (2e-5, 1e-6), "fp32_direct": (2e-5, 1e-6), "bf16x3": (1e-5, 1e-6),
+TOL = {"fp32": (2e-5, 1e-6), "fp32_direct": (2e-5, 1e-6), "bf16x3": (1e-5, 2e-6),
        "fp16": (5e-3, 5e-4)}
 
 
