@@ -30,9 +30,15 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def time_form(lib, torch, prob, projs, form, steps, warmup):
-    plan = lib.plan_for_problem(prob, projs, form)
+def time_form(lib, torch, prob, projs, form, steps, warmup, tune=True):
     t = lib.problem_tensors(prob)
+    tuned = None
+    if tune:
+        probe = lib.plan_for_problem(prob, projs, form)
+        tuned = probe.autotune(t, iters=2)
+        probe.close()
+        log(f"{form}: autotuner picked variant {tuned[0]} ({tuned[1]:.3f} ms cross term)")
+    plan = lib.plan_for_problem(prob, projs, form)
     plan.timing(max(steps, 1))
     for _ in range(warmup):
         plan.execute(t)
@@ -45,7 +51,7 @@ def time_form(lib, torch, prob, projs, form, steps, warmup):
     cross_ms = plan.timing_read(steps)
     res = dict(form=form, ms_per_step=(t1 - t0) * 1e3 / steps,
                cross_ms=sum(cross_ms) / len(cross_ms), variant=plan.tile(),
-               plan_bytes=plan.device_bytes())
+               plan_bytes=plan.device_bytes(), autotuned=tuned is not None)
     return res, t, plan
 
 
@@ -103,6 +109,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-orient", type=int, default=256)
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of "
@@ -124,7 +131,8 @@ def main():
     forms = args.forms.split(",")
     results, outs = {}, {}
     for form in forms:
-        r, t, plan = time_form(lib, torch, prob, projs, form, args.steps, args.warmup)
+        r, t, plan = time_form(lib, torch, prob, projs, form, args.steps, args.warmup,
+                               tune=not args.no_tune)
         r["evals_per_s"] = evals / (r["ms_per_step"] * 1e-3)
         flops = 4.0 * npix * evals
         r["cross_tflops"] = flops / (r["cross_ms"] * 1e-3) / 1e12

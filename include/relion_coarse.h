@@ -179,6 +179,23 @@ int rc_plan_tile(const rc_plan *plan, int *variant, int *tile_m, int *tile_n);
 int rc_plan_timing(rc_plan *plan, int n_slots);
 int rc_plan_timing_read(rc_plan *plan, float *ms, int n, int *n_written);
 
+/* hipGraph: capture one rc_plan_execute of desc on `stream` into a graph
+ * held by the plan (replacing an earlier one), then replay it with
+ * rc_plan_replay on any stream.  The captured graph holds the pointers of
+ * desc; replays read the buffers' current contents.  Timing slots are not
+ * recorded inside the graph. */
+int rc_plan_capture(rc_plan *plan, const rc_batch_desc *desc, void *stream);
+int rc_plan_replay(rc_plan *plan, void *stream);
+
+/* Autotuner: time every cross-term variant of desc's form on desc's shape
+ * (one plan per variant in turn, one warm-up and `iters` timed launches,
+ * HIP events on `stream`), keep the fastest in a process-wide table keyed
+ * by the shape (particles, classes, orientations, pixels, form, row groups)
+ * that later rc_plan_create calls consult, and return it.  Allocates and
+ * synchronises the stream; not for graph capture. */
+int rc_autotune(const rc_batch_desc *desc, void *stream, int iters, int *best_variant,
+                float *best_ms);
+
 /* Force the cross-term kernel variant of later rc_plan_create calls
  * (-1 = automatic).  Used by the autotuner. */
 int rc_set_variant_override(int variant);

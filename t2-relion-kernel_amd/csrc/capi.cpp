@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -206,6 +208,31 @@ static int scratch_for(size_t bytes, void **out)
 // ---------------------------------------------------------------------------
 static int g_variant_override = -1;
 
+struct TuneKey {
+    int P, C, O, npix, form, groups;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(P, C, O, npix, form, groups) <
+               std::tie(o.P, o.C, o.O, o.npix, o.form, o.groups);
+    }
+};
+static std::mutex g_tune_mu;
+static std::map<TuneKey, int> g_tuned;
+
+static int count_groups(const rc_batch_desc *d)
+{
+    int g = 0;
+    for (int i = 0; i < d->n_particles; ++i)
+        g += (d->host_trans_offset[i + 1] - d->host_trans_offset[i] + 15) / 16;
+    return g;
+}
+
+static TuneKey tune_key(const rc_batch_desc *d)
+{
+    return TuneKey{d->n_particles, d->n_classes, d->orientation_num, d->npix, d->form,
+                   count_groups(d)};
+}
+
 }  // namespace rc
 
 struct rc_plan {
@@ -230,6 +257,10 @@ struct rc_plan {
     long long *part_idx;
     std::vector<hipEvent_t> ev;             // timing slots: 2 per launch
     long long launches = 0;
+    bool capturing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int persist_blocks = 0;
 };
 
 namespace rc {
@@ -469,6 +500,53 @@ int rc_coarse_run(int translation_num, int orientation_num, int image_size, int 
                       workspace_bytes);
 }
 
+int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_variant,
+                float *best_ms)
+{
+    if (int st = validate_desc(d, true)) return st;
+    RC_REQUIRE(iters > 0 && iters <= 1000, RC_ERR_INVALID, "iters must be 1..1000");
+    const int n = variant_count(d->form, false);
+    const int saved = g_variant_override;
+    int best = -1;
+    float best_t = 0.f;
+    std::vector<float> ms(iters);
+    for (int v = 0; v < n; ++v) {
+        g_variant_override = v;
+        rc_plan *p = nullptr;
+        int st = rc_plan_create(d, &p);
+        if (st == RC_OK) st = rc_plan_timing(p, iters);
+        if (st == RC_OK) st = rc_plan_execute(p, d, stream);
+        if (st == RC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+            set_error("stream synchronisation failed while tuning");
+            st = RC_ERR_DEVICE;
+        }
+        if (st == RC_OK) p->launches = 0;
+        for (int k = 0; st == RC_OK && k < iters; ++k) st = rc_plan_execute(p, d, stream);
+        int got = 0;
+        if (st == RC_OK) st = rc_plan_timing_read(p, ms.data(), iters, &got);
+        if (p) rc_plan_destroy(p);
+        if (st != RC_OK) {
+            g_variant_override = saved;
+            return st;
+        }
+        float t = 0.f;
+        for (int k = 0; k < got; ++k) t += ms[k];
+        t /= std::max(got, 1);
+        if (best < 0 || t < best_t) {
+            best = v;
+            best_t = t;
+        }
+    }
+    g_variant_override = saved;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[tune_key(d)] = best;
+    }
+    if (best_variant) *best_variant = best;
+    if (best_ms) *best_ms = best_t;
+    return RC_OK;
+}
+
 int rc_set_variant_override(int variant)
 {
     g_variant_override = variant;
@@ -513,7 +591,12 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
     }
     gbeg[p->P] = (int)groups.size();
     p->n_groups = (int)groups.size();
-    int vi = g_variant_override >= 0 ? g_variant_override : default_variant(p->form, p->P, tmax);
+    int vi = g_variant_override;
+    if (vi < 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(tune_key(d));
+        vi = it != g_tuned.end() ? it->second : default_variant(p->form, p->P, tmax);
+    }
     const VariantInfo *info = variant_info(p->form, false, vi);
     if (!info) {
         delete p;
@@ -525,6 +608,14 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
     p->n_mtiles = (p->n_groups * 16 + info->tile_m - 1) / info->tile_m;
     p->ntiles_per_class = (p->O + info->tile_n - 1) / info->tile_n;
     p->parts_per_group = info->waves_n;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        const int per_cu = std::max(1, std::min(2, (160 * 1024) / std::max(info->lds_bytes, 1)));
+        p->persist_blocks = cus * per_cu;
+    }
     for (int c = 0; c < p->C; ++c) {
         const rc_projector *pr = d->projectors[c].proj;
         p->proj_dims.insert(p->proj_dims.end(), {pr->xdim, pr->ydim, pr->zdim, pr->inity,
@@ -614,15 +705,56 @@ int rc_plan_execute(rc_plan *p, const rc_batch_desc *d, void *stream_)
     const bool want_best = d->d_best_diff2 || d->d_best_index;
     a.part_min = want_best ? p->part_min : nullptr;
     a.part_idx = want_best ? p->part_idx : nullptr;
-    const int nslot = (int)p->ev.size() / 2;
+    a.persist_blocks = p->persist_blocks;
+    const int nslot = p->capturing ? 0 : (int)p->ev.size() / 2;
     if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot)], s));
     RC_HIP(launch_cross(p->form, false, p->variant, a, s));
-    if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
-    ++p->launches;
+    if (nslot) {
+        RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
+        ++p->launches;
+    }
     if (want_best)
         RC_HIP(launch_argmin(p->P, p->group_begin,
                              p->C * p->ntiles_per_class * p->parts_per_group, p->part_min,
                              p->part_idx, d->d_best_diff2, (long long *)d->d_best_index, s));
+    return RC_OK;
+}
+
+static void drop_graph(rc_plan *p)
+{
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    p->graph_exec = nullptr;
+    p->graph = nullptr;
+}
+
+int rc_plan_capture(rc_plan *p, const rc_batch_desc *d, void *stream)
+{
+    RC_REQUIRE(p, RC_ERR_INVALID, "NULL plan");
+    hipStream_t s = (hipStream_t)stream;
+    RC_REQUIRE(s != nullptr, RC_ERR_INVALID, "capture needs a non-default stream");
+    drop_graph(p);
+    RC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    p->capturing = true;
+    const int st = rc_plan_execute(p, d, stream);
+    p->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (st != RC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    RC_REQUIRE(e == hipSuccess && g, RC_ERR_DEVICE, "stream capture failed: %s",
+               hipGetErrorString(e));
+    p->graph = g;
+    RC_HIP(hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0));
+    return RC_OK;
+}
+
+int rc_plan_replay(rc_plan *p, void *stream)
+{
+    RC_REQUIRE(p && p->graph_exec, RC_ERR_INVALID, "plan has no captured graph");
+    RC_HIP(hipGraphLaunch(p->graph_exec, (hipStream_t)stream));
     return RC_OK;
 }
 
@@ -664,6 +796,7 @@ int rc_plan_destroy(rc_plan *p)
 {
     if (!p) return RC_OK;
     drop_events(p);
+    drop_graph(p);
     hipError_t e = p->mem ? hipFree(p->mem) : hipSuccess;
     delete p;
     RC_REQUIRE(e == hipSuccess, RC_ERR_DEVICE, "hipFree of the plan failed");

@@ -162,55 +162,63 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
 
 // ---------------------------------------------------------------------------
 // |P|^2 norms: pnorm[i][n] = sum_p half_corr[i][p] * pabs2[n][p]
-// block tile 32 particles x 64 columns, 256 threads, each 2 x 4 outputs
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void pnorm_kernel(int P, int N, int kpad,
-                                                    const float *__restrict__ hc,
-                                                    const float *__restrict__ pabs2,
-                                                    float *__restrict__ pnorm)
+// A small GEMM on the matrix cores (v_mfma_f32_16x16x4_f32): block tile
+// 64 particles x 64 columns, 4 waves of 32 x 32, k staged through LDS in
+// 32-pixel chunks.  A[i][k] = half_corr, B[k][n] = |P|^2.
+__global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
+                                                         const float *__restrict__ hc,
+                                                         const float *__restrict__ pabs2,
+                                                         float *__restrict__ pnorm)
 {
-    __shared__ float sA[32][33];
+    __shared__ float sA[64][33];
     __shared__ float sB[64][33];
-    const int i0 = blockIdx.y * 32, n0 = blockIdx.x * 64;
-    const int ty = threadIdx.x / 16, tx = threadIdx.x % 16;   // 16 x 16
-    float acc[2][4] = {};
+    const int i0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+    const int r16 = lane & 15, q = lane >> 4;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < kpad; k0 += 32) {
-        for (int u = threadIdx.x; u < 32 * 32; u += 256) {
-            const int r = u / 32, k = u % 32;
-            sA[r][k] = (i0 + r < P) ? hc[(size_t)(i0 + r) * kpad + k0 + k] : 0.f;
-        }
         for (int u = threadIdx.x; u < 64 * 32; u += 256) {
-            const int r = u / 32, k = u % 32;
+            const int r = u >> 5, k = u & 31;
+            sA[r][k] = (i0 + r < P) ? hc[(size_t)(i0 + r) * kpad + k0 + k] : 0.f;
             sB[r][k] = (n0 + r < N) ? pabs2[(size_t)(n0 + r) * kpad + k0 + k] : 0.f;
         }
         __syncthreads();
-#pragma unroll 8
-        for (int k = 0; k < 32; ++k) {
-            const float a0 = sA[ty][k], a1 = sA[ty + 16][k];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const float b = sB[tx + 16 * j][k];
-                acc[0][j] = fmaf(a0, b, acc[0][j]);
-                acc[1][j] = fmaf(a1, b, acc[1][j]);
-            }
+        for (int s = 0; s < 8; ++s) {
+            const int k = 4 * s + q;
+            const float a0 = sA[wr + r16][k], a1 = sA[wr + 16 + r16][k];
+            const float b0 = sB[wc + r16][k], b1 = sB[wc + 16 + r16][k];
+            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
         __syncthreads();
     }
+    // D[row = 4q + v][col = r16] of each 16 x 16 fragment
 #pragma unroll
-    for (int r = 0; r < 2; ++r)
+    for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int i = i0 + ty + 16 * r, n = n0 + tx + 16 * j;
-            if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[r][j];
-        }
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wr + 16 * a + 4 * q + v, n = n0 + wc + 16 * b + r16;
+                if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[a][b][v];
+            }
 }
 
 hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,
                         const float *pabs2, float *pnorm, hipStream_t s)
 {
-    dim3 grid((n_cols + 63) / 64, (n_particles + 31) / 32);
-    hipLaunchKernelGGL(pnorm_kernel, grid, dim3(256), 0, s, n_particles, n_cols, kpad, half_corr,
-                       pabs2, pnorm);
+    dim3 grid((n_cols + 63) / 64, (n_particles + 63) / 64);
+    hipLaunchKernelGGL(pnorm_mfma_kernel, grid, dim3(256), 0, s, n_particles, n_cols, kpad,
+                       half_corr, pabs2, pnorm);
     return hipGetLastError();
 }
 

@@ -87,6 +87,7 @@ struct CrossArgs {
     int n_classes, orientation_num;
     int ntiles_per_class;        // ceil(O / tile_n)
     int n_mtiles;
+    int persist_blocks;          // persistent variants: blocks in the grid
     // outputs
     const float *diff2_init;     // same layout as diff2, may be NULL, may alias diff2
     float *diff2;
@@ -102,6 +103,7 @@ struct VariantInfo {
     int waves_n;
     bool fused;
     int lds_bytes;
+    bool persistent;
 };
 
 int variant_count(int form, bool fused);

@@ -32,7 +32,7 @@ EXPORTS = [
     "rc_coarse_workspace_bytes", "rc_coarse_run", "rc_plan_create", "rc_plan_execute",
     "rc_plan_destroy", "rc_plan_device_bytes", "rc_plan_tile", "rc_set_variant_override",
     "rc_variant_count", "rc_variant_info", "rc_last_error", "rc_version", "rc_plan_timing",
-    "rc_plan_timing_read",
+    "rc_plan_timing_read", "rc_autotune", "rc_plan_capture", "rc_plan_replay",
 ]
 
 
@@ -91,7 +91,10 @@ def load():
     L.rc_plan_device_bytes.argtypes = [vp]
     L.rc_plan_device_bytes.restype = sz
     L.rc_plan_tile.argtypes = [vp, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.rc_autotune.argtypes = [C.POINTER(rc_batch_desc), vp, i, C.POINTER(i), fp]
     L.rc_plan_timing.argtypes = [vp, i]
+    L.rc_plan_capture.argtypes = [vp, C.POINTER(rc_batch_desc), vp]
+    L.rc_plan_replay.argtypes = [vp, vp]
     L.rc_plan_timing_read.argtypes = [vp, fp, i, C.POINTER(i)]
     L.rc_set_variant_override.argtypes = [i]
     L.rc_variant_count.argtypes = [i]
@@ -258,10 +261,28 @@ class BatchPlan:
         check(self.L.rc_plan_timing_read(self.handle, ms, n, C.byref(k)))
         return list(ms[:k.value])
 
+    def autotune(self, tensors, iters=3, stream=None):
+        """Time every variant on this shape; later plans of the shape use the
+        fastest.  Returns (variant, ms of the cross-term launch)."""
+        d = self._desc(tensors)
+        v, ms = C.c_int(), C.c_float()
+        check(self.L.rc_autotune(C.byref(d), _stream(stream), int(iters), C.byref(v),
+                                 C.byref(ms)))
+        return v.value, ms.value
+
     def execute(self, tensors, stream=None):
         d = self._desc(tensors)
         self._last_desc = d
         check(self.L.rc_plan_execute(self.handle, C.byref(d), _stream(stream)))
+
+    def capture(self, tensors, stream):
+        """Capture one execute into a hipGraph held by the plan."""
+        d = self._desc(tensors)
+        self._graph_desc = d
+        check(self.L.rc_plan_capture(self.handle, C.byref(d), _stream(stream)))
+
+    def replay(self, stream=None):
+        check(self.L.rc_plan_replay(self.handle, _stream(stream)))
 
     def close(self):
         if self.handle:

@@ -192,3 +192,59 @@ def test_batched_variants_agree(c1, lib, torch_cuda):
                                   f"variant {v} ({tm}x{tn}) particle {i}")
     finally:
         lib.set_variant_override(-1)
+
+
+def test_plan_graph_replay_matches_eager(c1, lib, torch_cuda):
+    """A captured plan replayed several times, with eager work and host
+    synchronisations between replays, reproduces the eager result exactly."""
+    torch = torch_cuda
+    prob, ref = c1
+    projs = [lib.Projector(v) for v in prob.vols]
+    plan = lib.plan_for_problem(prob, projs, "bf16x3")
+    t = lib.problem_tensors(prob)
+    plan.execute(t)
+    torch.cuda.synchronize()
+    eager = t["diff2"].clone()
+    eager_best = t["best_index"].clone()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.capture(t, s)
+    torch.cuda.synchronize()
+    for k in range(3):
+        t["diff2"].fill_(-1.0)
+        t["best_index"].fill_(-1)
+        torch.cuda.synchronize()
+        plan.replay(s)
+        s.synchronize()
+        assert torch.equal(t["diff2"], eager), k
+        assert torch.equal(t["best_index"], eager_best), k
+        plan.execute(t)          # eager work between replays
+        torch.cuda.synchronize()
+
+
+def test_batched_deterministic_alone_and_in_batch(c1, lib, torch_cuda):
+    """Each particle's diff2 is bit-identical whether it runs alone or inside
+    a larger batch (no atomics, no split-K)."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob, _ = c1
+    projs = [lib.Projector(v) for v in prob.vols]
+    full = lib.plan_for_problem(prob, projs, "bf16x3")
+    t = lib.problem_tensors(prob)
+    full.execute(t)
+    torch.cuda.synchronize()
+    d_full = t["diff2"].cpu().numpy()
+    off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
+    pb = prob.particles
+    for i in (0, 5):
+        one = synth.ParticleBatch(pb.real[i:i + 1], pb.imag[i:i + 1], pb.corr[i:i + 1],
+                                  pb.coor_xy, pb.npix, pb.full_npix, pb.true_rot[i:i + 1],
+                                  pb.true_shift[i:i + 1], pb.classes[i:i + 1], pb.snr)
+        tx, ty = prob.trans_of(i)
+        p1 = synth.CoarseProblem("one", prob.vols, prob.geom, prob.eulers,
+                                 np.array([0, tx.size], np.int32), tx, ty, one, prob.box)
+        plan = lib.plan_for_problem(p1, projs, "bf16x3")
+        t1 = lib.problem_tensors(p1)
+        plan.execute(t1)
+        torch.cuda.synchronize()
+        assert np.array_equal(t1["diff2"].cpu().numpy(), d_full[off[i]:off[i + 1]])
