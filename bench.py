@@ -98,6 +98,35 @@ def fp64_check(lib, torch, prob, outs, n_orient):
     return rep
 
 
+def serve_bench(n_requests, rate_hz, forms, slots, budget_gb):
+    """Streamed serving: Poisson arrivals of mixed-box micrograph requests,
+    continuous batching on one GPU; per-request latency from submission to
+    result, and aggregate evals/s, for each form on the same stream."""
+    from relion_amd import serving
+    t0 = time.perf_counter()
+    stream, cache = serving.make_stream(n_requests, rate_hz)
+    log(f"serving stream: {n_requests} requests at {rate_hz}/s generated in "
+        f"{time.perf_counter() - t0:.1f}s")
+    out = {}
+    for form in forms:
+        srv = serving.Server(form, hbm_budget=int(budget_gb * (1 << 30)),
+                             pinned_budget=4 << 30, max_inflight=slots)
+        # warm-up request (kernel loading) outside the measured stream
+        tk = srv.submit(stream[0][1])
+        srv.wait(tk, stream[0][1].P)
+        lat, wall = serving.run_stream(srv, stream)
+        rep = serving.summarize(lat, stream, wall)
+        rep["server"] = srv.stats()
+        srv.close()
+        out[form] = rep
+        log(f"serve {form}: p50 {rep['p50_ms']:.2f} ms p99 {rep['p99_ms']:.2f} ms, "
+            f"{rep['evals_per_s']:.3e} evals/s, peak inflight {rep['server']['peak_inflight']}")
+    out["workload"] = (f"{n_requests} requests, Poisson {rate_hz}/s, 20-400 particles, "
+                       "box {64,128,256} -> coarse {32,64,96}, 1-4 classes, 4608 orientations, "
+                       f"shift radius 3-6 px; {slots} slots, HBM budget {budget_gb} GiB")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +139,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--check-orient", type=int, default=256)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of "
@@ -141,6 +174,11 @@ def main():
         plan.close()
         log(f"{form}: {r['ms_per_step']:.3f} ms/step, cross {r['cross_ms']:.3f} ms, "
             f"{r['evals_per_s']:.3e} evals/s, {r['cross_tflops']:.1f} TFLOP/s, tile {r['variant']}")
+
+    streamed = None
+    if args.serve_requests > 0:
+        streamed = serve_bench(args.serve_requests, args.serve_rate, forms, args.serve_slots,
+                               args.serve_budget_gb)
 
     check = fp64_check(lib, torch, prob, outs, min(args.check_orient, prob.orientation_num))
     log(f"fp64 check: {check}")
@@ -177,6 +215,7 @@ def main():
                      "flops_per_launch": 4.0 * npix * evals},
         "forms": {f: {k: v for k, v in x.items()} for f, x in results.items()},
         "precision_vs_fp64": check,
+        "streamed": streamed,
         "cpu_baseline": cpu,
     }
     if "fp32" in results and head != "fp32":

@@ -92,8 +92,10 @@ int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ
  * Unfiltered image: image_size pixels whose (x, y) follow from the pixel
  * index (x = i % imgX, y = i / imgX, y -= imgY if y > maxR; diff2.cuh:182-188).
  * d_diff2s[o*T+t] is accumulated (+=).  trans_z and local_sqrtXi2 are
- * accepted and unused, as in the reference's 2D-data path.  do_CC != 0 or
- * data_is_3D != 0 -> RC_ERR_UNSUPPORTED.  Needs scratch device memory; it is
+ * accepted and unused, as in the reference's 2D-data path.  do_CC != 0
+ * writes the cross-correlation score -sum corr Re(P conj Xs) /
+ * sqrt(sum corr |P|^2) instead (diff2.cuh:691-809, overwriting, with that
+ * kernel's pixel-coordinate rule).  data_is_3D != 0 -> RC_ERR_UNSUPPORTED.  Needs scratch device memory; it is
  * taken from an internal per-device cache that grows on demand (the only
  * entry point that may allocate). */
 int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
@@ -149,6 +151,8 @@ typedef struct {
                            then [class][orientation][translation]                */
     double *d_best_diff2;    /* device out [n_particles] (float64) or NULL        */
     int64_t *d_best_index;   /* device out [n_particles] flat (c*O+o)*T_i+t, or NULL */
+    int do_cc;               /* 1: cross-correlation score (diff2.cuh:691-809) in
+                                d_diff2 (written; d_diff2_init ignored); min = best */
 } rc_batch_desc;
 
 typedef struct rc_plan rc_plan;
@@ -203,6 +207,62 @@ int rc_set_variant_override(int variant);
 /* Number of cross-term kernel variants and their tile shapes. */
 int rc_variant_count(int form);
 int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_gather);
+
+/* ---------------------------------------------------------------------- */
+/* Serving engine (no reference counterpart): continuous batching of       */
+/* coarse-search requests on one GPU                                       */
+/* ---------------------------------------------------------------------- */
+/* One worker thread owns the device.  Queued requests are admitted in
+ * arrival order as soon as one of max_inflight slots is free and the
+ * request's device memory (inputs, plan buffers, diff2 output) fits in the
+ * HBM budget; each slot runs its request on its own stream, so admitted
+ * requests overlap on the device.  Finished requests return their memory at
+ * once (or at rc_server_release when keep_diff2 is set).  Projector volumes
+ * stay device-resident: requests reference rc_projector objects that the
+ * caller keeps alive for the server's lifetime.  Results are deterministic:
+ * a request's diff2 is bit-identical alone and in any mix. */
+typedef struct rc_server rc_server;
+
+typedef struct {
+    size_t hbm_budget_bytes;     /* device arena for admitted requests          */
+    size_t pinned_budget_bytes;  /* pinned host staging for queued requests     */
+    int max_inflight;            /* requests on the device at once, 1..64       */
+    int form;                    /* rc_form of the cross term                    */
+} rc_server_config;
+
+typedef struct {
+    int n_particles, n_classes, orientation_num, npix, full_npix;
+    const rc_projector_kernel *projectors;   /* host [n_classes]              */
+    const int *trans_offset;                 /* host [n_particles+1], CSR     */
+    const float *trans_x, *trans_y;          /* host [trans_offset[P]]        */
+    const float *eulers;                     /* host [O*9]                    */
+    const float *real, *imag, *corr;         /* host [n_particles*full_npix]  */
+    const float *coor_xy;                    /* host [full_npix*2]            */
+    int keep_diff2;                          /* keep diff2 on the device until
+                                                rc_server_release            */
+} rc_request;
+
+typedef struct {
+    uint64_t completed;
+    int queued, inflight, peak_inflight;
+    size_t hbm_used, hbm_peak, hbm_budget;
+    double busy_s;               /* first admission to last completion          */
+} rc_server_counters;
+
+int rc_server_create(const rc_server_config *cfg, rc_server **out);
+/* Copies the request's host inputs into pinned staging and queues it;
+ * returns at once.  RC_ERR_NO_MEMORY when the staging is full,
+ * RC_ERR_LIMIT when the request can never fit the HBM budget. */
+int rc_server_submit(rc_server *s, const rc_request *req, uint64_t *ticket);
+/* Blocks until the request is done; copies the per-particle float64 minimum
+ * and flat argmin (c*O+o)*T_i+t; latency = submission to completion. */
+int rc_server_wait(rc_server *s, uint64_t ticket, double *best, int64_t *best_index,
+                   double *latency_ms);
+/* keep_diff2 requests: copy the diff2 block ([particle][class][o][t]) to host. */
+int rc_server_diff2(rc_server *s, uint64_t ticket, float *host_out, size_t count);
+int rc_server_release(rc_server *s, uint64_t ticket);
+int rc_server_stats(rc_server *s, rc_server_counters *out);
+int rc_server_destroy(rc_server *s);
 
 /* ---------------------------------------------------------------------- */
 const char *rc_last_error(void);

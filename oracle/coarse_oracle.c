@@ -254,3 +254,58 @@ int oracle_diff2_f32(const float *vre, const float *vim, int mdlX, int mdlY, int
     }
     return used;
 }
+
+/* ---------------------------------------------------------------------- */
+/* cross-correlation score (float64)                                      */
+/* ---------------------------------------------------------------------- */
+/* cuda_kernel_diff2_CC_coarse (diff2.cuh:691-809):
+ *   out[o*T+t] = - sum_p corr_p Re(P_op conj(Xs_tp)) / sqrt(sum_p corr_p |P_op|^2)
+ * (overwrites; the full corr, not corr/2).  Pixel coordinates follow the CC
+ * kernel's own rule: y > maxR maps to y - imgY when y >= imgY - maxR, else
+ * x = maxR (diff2.cuh:765-771); out[] is written, not accumulated. */
+void oracle_cc_f64(const float *vre, const float *vim, int mdlX, int mdlY, int mdlZ, int initY,
+                   int initZ, float pad, int maxR2_padded, int imgX, int imgY, int maxR,
+                   const float *eulers, int O, const float *trans_x, const float *trans_y, int T,
+                   const float *re, const float *im, const float *corr, int image_size,
+                   double *out)
+{
+    vol_t v = {vre, vim, mdlX, mdlY, mdlZ, initY, initZ};
+#pragma omp parallel
+    {
+        double *pr = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *pi = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *xs = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *ys = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        for (int p = 0; p < image_size; ++p) {
+            int x = p % imgX, y = p / imgX;
+            if (y > maxR) {
+                if (y >= imgY - maxR)
+                    y -= imgY;
+                else
+                    x = maxR;
+            }
+            xs[p] = x;
+            ys[p] = y;
+        }
+#pragma omp for schedule(dynamic, 4)
+        for (int o = 0; o < O; ++o) {
+            const float *e = eulers + 9 * (size_t)o;
+            double norm = 0.0;
+            for (int p = 0; p < image_size; ++p) {
+                project_f64(&v, (double)pad, maxR2_padded, e, xs[p], ys[p], pr + p, pi + p);
+                norm += (pr[p] * pr[p] + pi[p] * pi[p]) * (double)corr[p];
+            }
+            for (int t = 0; t < T; ++t) {
+                double w = 0.0;
+                for (int p = 0; p < image_size; ++p) {
+                    double phi = xs[p] * (double)trans_x[t] + ys[p] * (double)trans_y[t];
+                    double s = sin(phi), c = cos(phi);
+                    double xr = c * re[p] - s * im[p];
+                    double xi = c * im[p] + s * re[p];
+                    w += (pr[p] * xr + pi[p] * xi) * (double)corr[p];
+                }
+                out[(size_t)o * T + t] = -(w / sqrt(norm));
+            }
+        }
+    }
+}

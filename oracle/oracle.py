@@ -50,6 +50,10 @@ def lib():
                                        C.c_float, C.c_int, _f32p, C.c_int, _f32p, _f32p, C.c_int,
                                        _f32p, _f32p, _f32p, _f32p, C.c_int, C.c_int, _f32p, C.c_int]
         L.oracle_diff2_f32.restype = C.c_int
+        L.oracle_cc_f64.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, C.c_int,
+                                    _f32p, _f32p, C.c_int, _f32p, _f32p, _f32p, C.c_int, _f64p]
+        L.oracle_cc_f64.restype = None
         _LIB = L
     return _LIB
 
@@ -115,6 +119,17 @@ def diff2_f32(v, maxR2_padded, eulers, trans_x, trans_y, re, im, corr, coor_xy, 
                                   _c32(trans_x), _c32(trans_y), T, _c32(re), _c32(im),
                                   _c32(corr), coor, n, int(n_proj), out, int(threads))
     return out, used
+
+
+def cc_f64(v, maxR2_padded, imgX, imgY, maxR, eulers, trans_x, trans_y, re, im, corr):
+    """float64 cross-correlation score of one unfiltered image: ``[O, T]``."""
+    eulers = _c32(eulers).reshape(-1, 9)
+    O, T = eulers.shape[0], int(np.size(trans_x))
+    n = int(np.size(re))
+    out = np.zeros(O * T)
+    lib().oracle_cc_f64(*_vol_args(v), int(maxR2_padded), imgX, imgY, maxR, eulers.ravel(), O,
+                        _c32(trans_x), _c32(trans_y), T, _c32(re), _c32(im), _c32(corr), n, out)
+    return out.reshape(O, T)
 
 
 def problem_diff2(prob, i, cls=0, orient=None, init=None):

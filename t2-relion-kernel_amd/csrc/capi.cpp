@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "plan_internal.hpp"
 #include "rc_internal.hpp"
 
 namespace rc {
@@ -113,7 +114,7 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                       const float *trans_x, const float *trans_y, const float *real,
                       const float *imag, const rc_projector_kernel *pk, const float *corr,
                       const float *init, float *dest, hipStream_t stream, const float *coor_xy,
-                      void *ws, size_t ws_bytes)
+                      void *ws, size_t ws_bytes, bool cc = false)
 {
     if (int st = check_kernel_view(pk)) return st;
     RC_REQUIRE(T > 0 && O > 0 && npix > 0, RC_ERR_INVALID,
@@ -134,8 +135,9 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
     const int kpad = (int)align_up((size_t)npix, kChunk);
     ProjView v = view_of(*pk);
     RC_HIP(launch_prep(1, npix, full_npix, kpad, real, imag, corr, coor_xy, pk->imgX, pk->imgY,
-                       pk->maxR, w.xw, w.coords, w.xnorm, w.half_corr, w.ascale, stream));
+                       pk->maxR, w.xw, w.coords, w.xnorm, w.half_corr, w.ascale, stream, cc));
     if (form == RC_FORM_FP32_DIRECT) {
+        RC_REQUIRE(!cc, RC_ERR_UNSUPPORTED, "the direct form has no cross-correlation mode");
         RC_REQUIRE(T <= 1024, RC_ERR_LIMIT, "direct form supports at most 1024 translations");
         RC_HIP(launch_direct(v, eulers, O, trans_x, trans_y, T, w.coords, real, imag, corr, npix,
                              full_npix, init, dest, stream));
@@ -168,6 +170,7 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
     a.diff2 = dest;
     a.part_min = nullptr;
     a.part_idx = nullptr;
+    a.cc = cc ? 1 : 0;
     RC_HIP(launch_cross(form, true, vi, a, stream));
     return RC_OK;
 }
@@ -235,33 +238,6 @@ static TuneKey tune_key(const rc_batch_desc *d)
 
 }  // namespace rc
 
-struct rc_plan {
-    int P, C, O, npix, full_npix, form, kpad, nkc;
-    int variant;
-    rc::VariantInfo info;
-    std::vector<int> trans_offset;          // host copy
-    std::vector<int> proj_dims;             // per class: 7 ints for shape checks
-    int n_groups, n_mtiles, ntiles_per_class, parts_per_group;
-    unsigned char *mem = nullptr;
-    size_t bytes = 0;
-    int2 *groups;
-    int *group_begin;
-    rc::ProjView *vols;
-    float *bscale;
-    float2 *xw, *coords;
-    double *xnorm;
-    float *half_corr, *ascale;
-    unsigned char *proj;
-    float *pabs2, *pnorm;
-    double *part_min;
-    long long *part_idx;
-    std::vector<hipEvent_t> ev;             // timing slots: 2 per launch
-    long long launches = 0;
-    bool capturing = false;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int persist_blocks = 0;
-};
 
 namespace rc {
 
@@ -276,7 +252,7 @@ static int default_variant(int form, int P, int Tmax)
     }
 }
 
-static int validate_desc(const rc_batch_desc *d, bool need_ptrs)
+int validate_batch(const rc_batch_desc *d, bool need_ptrs)
 {
     RC_REQUIRE(d, RC_ERR_INVALID, "NULL batch descriptor");
     RC_REQUIRE(d->n_particles > 0 && d->n_classes > 0 && d->orientation_num > 0 && d->npix > 0,
@@ -303,7 +279,7 @@ static int validate_desc(const rc_batch_desc *d, bool need_ptrs)
     return RC_OK;
 }
 
-static size_t plan_layout(rc_plan *p, unsigned char *base)
+size_t plan_bind(rc_plan *p, unsigned char *base)
 {
     Carve c{base};
     const size_t nt_total = (size_t)p->C * p->ntiles_per_class;
@@ -322,6 +298,159 @@ static size_t plan_layout(rc_plan *p, unsigned char *base)
     p->part_min = c.take<double>((size_t)p->n_groups * nt_total * p->parts_per_group);
     p->part_idx = c.take<long long>((size_t)p->n_groups * nt_total * p->parts_per_group);
     return align_up(c.off, 256);
+}
+
+}  // namespace rc
+
+namespace rc {
+
+int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
+{
+    if (int st = validate_batch(d, false)) return st;
+    p->P = d->n_particles;
+    p->C = d->n_classes;
+    p->O = d->orientation_num;
+    p->npix = d->npix;
+    p->full_npix = d->full_npix;
+    p->form = d->form;
+    p->kpad = (int)align_up((size_t)d->npix, kChunk);
+    p->nkc = p->kpad / kChunk;
+    p->trans_offset.assign(d->host_trans_offset, d->host_trans_offset + d->n_particles + 1);
+    int tmax = 0;
+    tab.groups.clear();
+    tab.group_begin.assign(p->P + 1, 0);
+    for (int i = 0; i < p->P; ++i) {
+        const int T = p->trans_offset[i + 1] - p->trans_offset[i];
+        tmax = std::max(tmax, T);
+        tab.group_begin[i] = (int)tab.groups.size();
+        for (int t0 = 0; t0 < T; t0 += 16) tab.groups.push_back(make_int2(i, t0));
+    }
+    tab.group_begin[p->P] = (int)tab.groups.size();
+    p->n_groups = (int)tab.groups.size();
+    int vi = g_variant_override;
+    if (vi < 0) {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(tune_key(d));
+        vi = it != g_tuned.end() ? it->second : default_variant(p->form, p->P, tmax);
+    }
+    const VariantInfo *info = variant_info(p->form, false, vi);
+    RC_REQUIRE(info, RC_ERR_INVALID, "variant %d does not exist for form %d", vi, d->form);
+    p->variant = vi;
+    p->info = *info;
+    p->n_mtiles = (p->n_groups * 16 + info->tile_m - 1) / info->tile_m;
+    p->ntiles_per_class = (p->O + info->tile_n - 1) / info->tile_n;
+    p->parts_per_group = info->waves_n;
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        const int per_cu = std::max(1, std::min(2, (160 * 1024) / std::max(info->lds_bytes, 1)));
+        p->persist_blocks = cus * per_cu;
+    }
+    p->proj_dims.clear();
+    tab.views.resize(p->C);
+    tab.scales.resize(p->C);
+    for (int c = 0; c < p->C; ++c) {
+        const rc_projector *pr = d->projectors[c].proj;
+        p->proj_dims.insert(p->proj_dims.end(), {pr->xdim, pr->ydim, pr->zdim, pr->inity,
+                                                 pr->initz, d->projectors[c].maxR2_padded,
+                                                 (int)(pr->pad * 1000)});
+        tab.views[c] = view_of(d->projectors[c]);
+        tab.scales[c] = pow2_scale_host(pr->max_abs);
+    }
+    return RC_OK;
+}
+
+size_t plan_table_bytes(const PlanTables &tab)
+{
+    return align_up(tab.groups.size() * sizeof(int2), 256) +
+           align_up(tab.group_begin.size() * sizeof(int), 256) +
+           align_up(tab.views.size() * sizeof(ProjView), 256) +
+           align_up(tab.scales.size() * sizeof(float), 256);
+}
+
+int plan_upload_async(rc_plan *p, const PlanTables &tab, unsigned char *staging, hipStream_t s)
+{
+    size_t off = 0;
+    auto put = [&](void *dst, const void *src, size_t n) -> hipError_t {
+        memcpy(staging + off, src, n);
+        hipError_t e = hipMemcpyAsync(dst, staging + off, n, hipMemcpyHostToDevice, s);
+        off += align_up(n, 256);
+        return e;
+    };
+    RC_HIP(put(p->groups, tab.groups.data(), tab.groups.size() * sizeof(int2)));
+    RC_HIP(put(p->group_begin, tab.group_begin.data(), tab.group_begin.size() * sizeof(int)));
+    RC_HIP(put(p->vols, tab.views.data(), tab.views.size() * sizeof(ProjView)));
+    RC_HIP(put(p->bscale, tab.scales.data(), tab.scales.size() * sizeof(float)));
+    return RC_OK;
+}
+
+int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
+{
+    RC_REQUIRE(p, RC_ERR_INVALID, "NULL plan");
+    if (int st = validate_batch(d, true)) return st;
+    RC_REQUIRE(d->n_particles == p->P && d->n_classes == p->C && d->orientation_num == p->O &&
+                   d->npix == p->npix && d->full_npix == p->full_npix && d->form == p->form,
+               RC_ERR_INVALID, "batch shape differs from the plan");
+    RC_REQUIRE(std::equal(p->trans_offset.begin(), p->trans_offset.end(), d->host_trans_offset),
+               RC_ERR_INVALID, "per-particle translation counts differ from the plan");
+    for (int c = 0; c < p->C; ++c) {
+        const rc_projector *pr = d->projectors[c].proj;
+        const int *dims = &p->proj_dims[7 * c];
+        RC_REQUIRE(dims[0] == pr->xdim && dims[1] == pr->ydim && dims[2] == pr->zdim &&
+                       dims[3] == pr->inity && dims[4] == pr->initz &&
+                       dims[5] == d->projectors[c].maxR2_padded,
+                   RC_ERR_INVALID, "projector %d differs from the plan", c);
+    }
+    const rc_projector_kernel &k0 = d->projectors[0];
+    RC_HIP(launch_prep(p->P, p->npix, p->full_npix, p->kpad, d->d_real, d->d_imag, d->d_corr,
+                       d->d_coor_xy, k0.imgX, k0.imgY, k0.maxR, p->xw, p->coords, p->xnorm,
+                       p->half_corr, p->ascale, s, d->do_cc != 0));
+    RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
+                          p->npix, p->kpad, p->proj, p->pabs2, s));
+    RC_HIP(launch_pnorm(p->P, p->C * p->O, p->kpad, p->half_corr, p->pabs2, p->pnorm, s));
+    CrossArgs a{};
+    a.groups = p->groups;
+    a.n_groups = p->n_groups;
+    a.trans_offset = d->d_trans_offset;
+    a.n_trans = 0;
+    a.trans_x = d->d_trans_x;
+    a.trans_y = d->d_trans_y;
+    a.xw = p->xw;
+    a.coords = p->coords;
+    a.xnorm = p->xnorm;
+    a.ascale = p->ascale;
+    a.kpad = p->kpad;
+    a.npix = p->npix;
+    a.proj = p->proj;
+    a.pnorm = p->pnorm;
+    a.bscale = p->bscale;
+    a.half_corr = p->half_corr;
+    a.eulers = d->d_eulers;
+    a.n_classes = p->C;
+    a.orientation_num = p->O;
+    a.ntiles_per_class = p->ntiles_per_class;
+    a.n_mtiles = p->n_mtiles;
+    a.diff2_init = d->d_diff2_init;
+    a.diff2 = d->d_diff2;
+    const bool want_best = d->d_best_diff2 || d->d_best_index;
+    a.part_min = want_best ? p->part_min : nullptr;
+    a.part_idx = want_best ? p->part_idx : nullptr;
+    a.persist_blocks = p->persist_blocks;
+    a.cc = d->do_cc ? 1 : 0;
+    const int nslot = p->capturing ? 0 : (int)p->ev.size() / 2;
+    if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot)], s));
+    RC_HIP(launch_cross(p->form, false, p->variant, a, s));
+    if (nslot) {
+        RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
+        ++p->launches;
+    }
+    if (want_best)
+        RC_HIP(launch_argmin(p->P, p->group_begin,
+                             p->C * p->ntiles_per_class * p->parts_per_group, p->part_min,
+                             p->part_idx, d->d_best_diff2, (long long *)d->d_best_index, s));
+    return RC_OK;
 }
 
 }  // namespace rc
@@ -456,7 +585,6 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
 {
     (void)d_trans_z;
     (void)local_sqrtXi2;
-    RC_REQUIRE(!do_CC, RC_ERR_UNSUPPORTED, "cross-correlation mode (do_CC) is not implemented");
     RC_REQUIRE(!data_is_3D, RC_ERR_UNSUPPORTED, "3D data (data_is_3D) is not implemented");
     if (int st = check_kernel_view(projector)) return st;
     RC_REQUIRE(orientation_num > 0 && translation_num > 0 && image_size > 0 &&
@@ -471,7 +599,7 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
     return run_single((int)translation_num, (int)orientation_num, (int)image_size,
                       (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real,
                       d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
-                      (hipStream_t)stream, nullptr, ws, w.bytes);
+                      (hipStream_t)stream, nullptr, ws, w.bytes, do_CC != 0);
 }
 
 size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int image_size,
@@ -503,7 +631,7 @@ int rc_coarse_run(int translation_num, int orientation_num, int image_size, int 
 int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_variant,
                 float *best_ms)
 {
-    if (int st = validate_desc(d, true)) return st;
+    if (int st = validate_batch(d, true)) return st;
     RC_REQUIRE(iters > 0 && iters <= 1000, RC_ERR_INVALID, "iters must be 1..1000");
     const int n = variant_count(d->form, false);
     const int saved = g_variant_override;
@@ -569,80 +697,27 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
 {
     RC_REQUIRE(out, RC_ERR_INVALID, "out is NULL");
     *out = nullptr;
-    if (int st = validate_desc(d, false)) return st;
     rc_plan *p = new rc_plan{};
-    p->P = d->n_particles;
-    p->C = d->n_classes;
-    p->O = d->orientation_num;
-    p->npix = d->npix;
-    p->full_npix = d->full_npix;
-    p->form = d->form;
-    p->kpad = (int)align_up((size_t)d->npix, kChunk);
-    p->nkc = p->kpad / kChunk;
-    p->trans_offset.assign(d->host_trans_offset, d->host_trans_offset + d->n_particles + 1);
-    int tmax = 0;
-    std::vector<int2> groups;
-    std::vector<int> gbeg(p->P + 1, 0);
-    for (int i = 0; i < p->P; ++i) {
-        const int T = p->trans_offset[i + 1] - p->trans_offset[i];
-        tmax = std::max(tmax, T);
-        gbeg[i] = (int)groups.size();
-        for (int t0 = 0; t0 < T; t0 += 16) groups.push_back(make_int2(i, t0));
-    }
-    gbeg[p->P] = (int)groups.size();
-    p->n_groups = (int)groups.size();
-    int vi = g_variant_override;
-    if (vi < 0) {
-        std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tuned.find(tune_key(d));
-        vi = it != g_tuned.end() ? it->second : default_variant(p->form, p->P, tmax);
-    }
-    const VariantInfo *info = variant_info(p->form, false, vi);
-    if (!info) {
+    PlanTables tab;
+    if (int st = plan_setup(p, d, tab)) {
         delete p;
-        set_error("variant %d does not exist for form %d", vi, d->form);
-        return RC_ERR_INVALID;
+        return st;
     }
-    p->variant = vi;
-    p->info = *info;
-    p->n_mtiles = (p->n_groups * 16 + info->tile_m - 1) / info->tile_m;
-    p->ntiles_per_class = (p->O + info->tile_n - 1) / info->tile_n;
-    p->parts_per_group = info->waves_n;
-    {
-        int dev = 0, cus = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-            cus = 256;
-        const int per_cu = std::max(1, std::min(2, (160 * 1024) / std::max(info->lds_bytes, 1)));
-        p->persist_blocks = cus * per_cu;
-    }
-    for (int c = 0; c < p->C; ++c) {
-        const rc_projector *pr = d->projectors[c].proj;
-        p->proj_dims.insert(p->proj_dims.end(), {pr->xdim, pr->ydim, pr->zdim, pr->inity,
-                                                 pr->initz, d->projectors[c].maxR2_padded,
-                                                 (int)(pr->pad * 1000)});
-    }
-    p->bytes = plan_layout(p, nullptr);
+    p->bytes = plan_bind(p, nullptr);
     if (hipMalloc(&p->mem, p->bytes) != hipSuccess) {
         const size_t b = p->bytes;
         delete p;
         set_error("hipMalloc of %zu bytes for the batch plan failed", b);
         return RC_ERR_NO_MEMORY;
     }
-    plan_layout(p, p->mem);
-    std::vector<ProjView> views(p->C);
-    std::vector<float> scales(p->C);
-    for (int c = 0; c < p->C; ++c) {
-        views[c] = view_of(d->projectors[c]);
-        scales[c] = pow2_scale_host(d->projectors[c].proj->max_abs);
-    }
-    if (hipMemcpy(p->groups, groups.data(), groups.size() * sizeof(int2),
+    plan_bind(p, p->mem);
+    if (hipMemcpy(p->groups, tab.groups.data(), tab.groups.size() * sizeof(int2),
                   hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->group_begin, gbeg.data(), gbeg.size() * sizeof(int),
+        hipMemcpy(p->group_begin, tab.group_begin.data(), tab.group_begin.size() * sizeof(int),
                   hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->vols, views.data(), views.size() * sizeof(ProjView),
+        hipMemcpy(p->vols, tab.views.data(), tab.views.size() * sizeof(ProjView),
                   hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(p->bscale, scales.data(), scales.size() * sizeof(float),
+        hipMemcpy(p->bscale, tab.scales.data(), tab.scales.size() * sizeof(float),
                   hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(p->mem);
         delete p;
@@ -653,71 +728,9 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
     return RC_OK;
 }
 
-int rc_plan_execute(rc_plan *p, const rc_batch_desc *d, void *stream_)
+int rc_plan_execute(rc_plan *p, const rc_batch_desc *d, void *stream)
 {
-    RC_REQUIRE(p, RC_ERR_INVALID, "NULL plan");
-    if (int st = validate_desc(d, true)) return st;
-    RC_REQUIRE(d->n_particles == p->P && d->n_classes == p->C && d->orientation_num == p->O &&
-                   d->npix == p->npix && d->full_npix == p->full_npix && d->form == p->form,
-               RC_ERR_INVALID, "batch shape differs from the plan");
-    RC_REQUIRE(std::equal(p->trans_offset.begin(), p->trans_offset.end(), d->host_trans_offset),
-               RC_ERR_INVALID, "per-particle translation counts differ from the plan");
-    for (int c = 0; c < p->C; ++c) {
-        const rc_projector *pr = d->projectors[c].proj;
-        const int *dims = &p->proj_dims[7 * c];
-        RC_REQUIRE(dims[0] == pr->xdim && dims[1] == pr->ydim && dims[2] == pr->zdim &&
-                       dims[3] == pr->inity && dims[4] == pr->initz &&
-                       dims[5] == d->projectors[c].maxR2_padded,
-                   RC_ERR_INVALID, "projector %d differs from the plan", c);
-    }
-    hipStream_t s = (hipStream_t)stream_;
-    const rc_projector_kernel &k0 = d->projectors[0];
-    RC_HIP(launch_prep(p->P, p->npix, p->full_npix, p->kpad, d->d_real, d->d_imag, d->d_corr,
-                       d->d_coor_xy, k0.imgX, k0.imgY, k0.maxR, p->xw, p->coords, p->xnorm,
-                       p->half_corr, p->ascale, s));
-    RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
-                          p->npix, p->kpad, p->proj, p->pabs2, s));
-    RC_HIP(launch_pnorm(p->P, p->C * p->O, p->kpad, p->half_corr, p->pabs2, p->pnorm, s));
-    CrossArgs a{};
-    a.groups = p->groups;
-    a.n_groups = p->n_groups;
-    a.trans_offset = d->d_trans_offset;
-    a.n_trans = 0;
-    a.trans_x = d->d_trans_x;
-    a.trans_y = d->d_trans_y;
-    a.xw = p->xw;
-    a.coords = p->coords;
-    a.xnorm = p->xnorm;
-    a.ascale = p->ascale;
-    a.kpad = p->kpad;
-    a.npix = p->npix;
-    a.proj = p->proj;
-    a.pnorm = p->pnorm;
-    a.bscale = p->bscale;
-    a.half_corr = p->half_corr;
-    a.eulers = d->d_eulers;
-    a.n_classes = p->C;
-    a.orientation_num = p->O;
-    a.ntiles_per_class = p->ntiles_per_class;
-    a.n_mtiles = p->n_mtiles;
-    a.diff2_init = d->d_diff2_init;
-    a.diff2 = d->d_diff2;
-    const bool want_best = d->d_best_diff2 || d->d_best_index;
-    a.part_min = want_best ? p->part_min : nullptr;
-    a.part_idx = want_best ? p->part_idx : nullptr;
-    a.persist_blocks = p->persist_blocks;
-    const int nslot = p->capturing ? 0 : (int)p->ev.size() / 2;
-    if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot)], s));
-    RC_HIP(launch_cross(p->form, false, p->variant, a, s));
-    if (nslot) {
-        RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
-        ++p->launches;
-    }
-    if (want_best)
-        RC_HIP(launch_argmin(p->P, p->group_begin,
-                             p->C * p->ntiles_per_class * p->parts_per_group, p->part_min,
-                             p->part_idx, d->d_best_diff2, (long long *)d->d_best_index, s));
-    return RC_OK;
+    return plan_run(p, d, (hipStream_t)stream);
 }
 
 static void drop_graph(rc_plan *p)

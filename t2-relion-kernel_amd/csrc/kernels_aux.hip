@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void prep_kernel(int npix, int full_npix, int 
                                                    float2 *__restrict__ coords,
                                                    double *__restrict__ xnorm,
                                                    float *__restrict__ half_corr,
-                                                   float *__restrict__ ascale)
+                                                   float *__restrict__ ascale, int cc_coords)
 {
     const int i = blockIdx.x;
     const size_t src = (size_t)i * full_npix;
@@ -54,7 +54,14 @@ __global__ __launch_bounds__(256) void prep_kernel(int npix, int full_npix, int 
                     cxy = make_float2(coor_xy[2 * p], coor_xy[2 * p + 1]);
                 } else {
                     int x = p % imgX, y = p / imgX;
-                    if (y > maxR) y -= imgY;
+                    if (y > maxR) {
+                        // the CC kernel parks rows between the two half-planes at
+                        // x = maxR (diff2.cuh:765-771); the diff2 kernels wrap them
+                        if (!cc_coords || y >= imgY - maxR)
+                            y -= imgY;
+                        else
+                            x = maxR;
+                    }
                     cxy = make_float2((float)x, (float)y);
                 }
             }
@@ -86,11 +93,11 @@ __global__ __launch_bounds__(256) void prep_kernel(int npix, int full_npix, int 
 hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const float *real,
                        const float *imag, const float *corr, const float *coor_xy, int imgX,
                        int imgY, int maxR, float2 *xw, float2 *coords, double *xnorm,
-                       float *half_corr, float *ascale, hipStream_t s)
+                       float *half_corr, float *ascale, hipStream_t s, bool cc_coords)
 {
     hipLaunchKernelGGL(prep_kernel, dim3(n_particles), dim3(256), 0, s, npix, full_npix, kpad,
                        real, imag, corr, coor_xy, imgX, imgY, maxR, xw, coords, xnorm, half_corr,
-                       ascale);
+                       ascale, (int)cc_coords);
     return hipGetLastError();
 }
 

@@ -32,6 +32,19 @@ namespace rc {
 
 constexpr double kInv2Pi = 0.15915494309189533577;
 
+// Score of one (row, column) in float64 from the cross term c = sum w
+// Re(P conj(Xs)) and the norms pn = sum w/2 |P|^2, xn = sum w/2 |X|^2:
+//   diff2:  init + pn + xn - c   (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:673-685)
+//   CC:     -c / sqrt(2 pn), written, not accumulated (diff2.cuh:797-805)
+__device__ __forceinline__ double assemble(const CrossArgs &a, float pn, double xn, float c,
+                                           size_t k)
+{
+    if (a.cc) return -(double)c / sqrt(2.0 * (double)pn);
+    double d = (double)pn + xn - (double)c;
+    if (a.diff2_init) d += (double)a.diff2_init[k];
+    return d;
+}
+
 template <int FORM, int WM, int WN, int FM, int FN, bool FUSED>
 struct CrossCfg {
     static constexpr int kThreads = 64 * WM * WN;
@@ -336,8 +349,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
                 const int t = g_t0[fa] + 4 * q + v;
                 if (t >= T) continue;
                 const size_t k = out_base + (size_t)o * T + t;
-                double d = (double)pn + xn - (double)(acc[fa][fb][v] * inv);
-                if (a.diff2_init) d += (double)a.diff2_init[k];
+                const double d = assemble(a, pn, xn, acc[fa][fb][v] * inv, k);
                 a.diff2[k] = (float)d;
                 const long long flat = (long long)(cls * O + o) * T + t;
                 if (d < best || (d == best && flat < best_k)) {
@@ -674,8 +686,7 @@ __device__ __forceinline__ void pipe_tile(const CrossArgs &a, int L, unsigned ch
                 const int t = g_t0[fa] + 4 * q + v;
                 if (t >= T) continue;
                 const size_t k = out_base + (size_t)o * T + t;
-                double d = (double)pn + xn - (double)(acc[fa][fb][v] * inv);
-                if (a.diff2_init) d += (double)a.diff2_init[k];
+                const double d = assemble(a, pn, xn, acc[fa][fb][v] * inv, k);
                 a.diff2[k] = (float)d;
                 const long long flat = (long long)(cls * O + o) * T + t;
                 if (d < best || (d == best && flat < best_k)) {

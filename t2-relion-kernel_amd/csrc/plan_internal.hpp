@@ -1,0 +1,60 @@
+// Internal view of a batched plan, shared by the C ABI (capi.cpp) and the
+// serving engine (server.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "rc_internal.hpp"
+
+struct rc_plan {
+    int P, C, O, npix, full_npix, form, kpad, nkc;
+    int variant;
+    rc::VariantInfo info;
+    std::vector<int> trans_offset;          // host copy
+    std::vector<int> proj_dims;             // per class: 7 ints for shape checks
+    int n_groups, n_mtiles, ntiles_per_class, parts_per_group;
+    unsigned char *mem = nullptr;
+    size_t bytes = 0;
+    int2 *groups;
+    int *group_begin;
+    rc::ProjView *vols;
+    float *bscale;
+    float2 *xw, *coords;
+    double *xnorm;
+    float *half_corr, *ascale;
+    unsigned char *proj;
+    float *pabs2, *pnorm;
+    double *part_min;
+    long long *part_idx;
+    std::vector<hipEvent_t> ev;             // timing slots: 2 per launch
+    long long launches = 0;
+    bool capturing = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int persist_blocks = 0;
+};
+
+namespace rc {
+
+// Host-side planning: shapes, row groups, variant, tables to upload.
+struct PlanTables {
+    std::vector<int2> groups;
+    std::vector<int> group_begin;
+    std::vector<ProjView> views;
+    std::vector<float> scales;
+};
+int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab);
+// Lay the plan's device buffers out from `mem` (NULL: only size them);
+// returns the bytes needed.
+size_t plan_bind(rc_plan *p, unsigned char *mem);
+// Asynchronous upload of the tables from pinned host memory `staging` of
+// at least plan_table_bytes(tab) bytes.
+size_t plan_table_bytes(const PlanTables &tab);
+int plan_upload_async(rc_plan *p, const PlanTables &tab, unsigned char *staging, hipStream_t s);
+// Validate desc against the plan and launch the kernels on s.
+int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s);
+int validate_batch(const rc_batch_desc *d, bool need_ptrs);
+
+}  // namespace rc

@@ -88,6 +88,7 @@ struct CrossArgs {
     int ntiles_per_class;        // ceil(O / tile_n)
     int n_mtiles;
     int persist_blocks;          // persistent variants: blocks in the grid
+    int cc;                      // cross-correlation score instead of diff2
     // outputs
     const float *diff2_init;     // same layout as diff2, may be NULL, may alias diff2
     float *diff2;
@@ -114,7 +115,7 @@ hipError_t launch_cross(int form, bool fused, int idx, const CrossArgs &a, hipSt
 hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const float *real,
                        const float *imag, const float *corr, const float *coor_xy, int imgX,
                        int imgY, int maxR, float2 *xw, float2 *coords, double *xnorm,
-                       float *half_corr, float *ascale, hipStream_t s);
+                       float *half_corr, float *ascale, hipStream_t s, bool cc_coords = false);
 
 // Projections of every (class, orientation, pixel) into the encoded layout.
 hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,

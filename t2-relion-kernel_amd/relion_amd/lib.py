@@ -33,6 +33,8 @@ EXPORTS = [
     "rc_plan_destroy", "rc_plan_device_bytes", "rc_plan_tile", "rc_set_variant_override",
     "rc_variant_count", "rc_variant_info", "rc_last_error", "rc_version", "rc_plan_timing",
     "rc_plan_timing_read", "rc_autotune", "rc_plan_capture", "rc_plan_replay",
+    "rc_server_create", "rc_server_submit", "rc_server_wait", "rc_server_diff2",
+    "rc_server_release", "rc_server_stats", "rc_server_destroy",
 ]
 
 
@@ -56,7 +58,8 @@ class rc_batch_desc(C.Structure):
                 ("d_trans_y", C.c_void_p), ("d_eulers", C.c_void_p), ("d_real", C.c_void_p),
                 ("d_imag", C.c_void_p), ("d_corr", C.c_void_p), ("d_coor_xy", C.c_void_p),
                 ("d_diff2_init", C.c_void_p), ("d_diff2", C.c_void_p),
-                ("d_best_diff2", C.c_void_p), ("d_best_index", C.c_void_p)]
+                ("d_best_diff2", C.c_void_p), ("d_best_index", C.c_void_p),
+                ("do_cc", C.c_int)]
 
 
 _lib = None
@@ -220,13 +223,14 @@ class BatchPlan:
     translations, one plan per shape."""
 
     def __init__(self, projector_kernels, trans_offset, n_particles, orientation_num, npix,
-                 full_npix, form):
+                 full_npix, form, cc=False):
         self.L = load()
         self.form = FORMS[form] if isinstance(form, str) else form
         self.pk = (rc_projector_kernel * len(projector_kernels))(*projector_kernels)
         self.trans_offset = np.ascontiguousarray(trans_offset, np.int32)
         self.P, self.C, self.O = n_particles, len(projector_kernels), orientation_num
         self.npix, self.full = npix, full_npix
+        self.cc = bool(cc)
         self.handle = C.c_void_p()
         d = self._desc(dict())
         check(self.L.rc_plan_create(C.byref(d), C.byref(self.handle)))
@@ -237,6 +241,7 @@ class BatchPlan:
         d.npix, d.full_npix, d.form = self.npix, self.full, self.form
         d.projectors = self.pk
         d.host_trans_offset = self.trans_offset.ctypes.data_as(C.POINTER(C.c_int))
+        d.do_cc = int(self.cc)
         for k in ("trans_offset", "trans_x", "trans_y", "eulers", "real", "imag", "corr",
                   "coor_xy", "diff2_init", "diff2", "best_diff2", "best_index"):
             v = t.get(k)
@@ -317,9 +322,9 @@ def problem_tensors(prob, device="cuda"):
         best_index=torch.empty(prob.n_particles, dtype=torch.int64, device=device))
 
 
-def plan_for_problem(prob, projectors, form):
+def plan_for_problem(prob, projectors, form, cc=False):
     g = prob.geom
     pks = [p.kernel(g.imgX, g.imgY, g.maxR) for p in projectors]
     pb = prob.particles
     return BatchPlan(pks, prob.trans_offset, prob.n_particles, prob.orientation_num, pb.npix,
-                     pb.full_npix, form)
+                     pb.full_npix, form, cc=cc)

@@ -34,7 +34,8 @@ import numpy as np
 
 __all__ = [
     "ProjectorVolume", "CoarseGeometry", "ParticleBatch", "CoarseProblem",
-    "blob_map", "fourier_volume", "healpix_centres", "euler_zyz",
+    "blob_map", "blob_params", "fourier_volume", "fourier_volume_analytic",
+    "healpix_centres", "euler_zyz",
     "orientation_set", "translation_set", "pixel_coords", "filter_order",
     "project_np", "make_particles", "make_problem", "CONFIGS",
 ]
@@ -51,19 +52,9 @@ def blob_map(n: int, n_blobs: int = 20, seed: int = 0,
     ``20 * n / 64``) around the box centre ``n/2``; sigmas ``U(sigma_range)``
     voxels; amplitudes ``U(0.5, 1)``.
     """
-    rng = np.random.default_rng(seed)
-    if centre_radius is None:
-        centre_radius = 20.0 * n / 64.0
     grid = np.arange(n, dtype=np.float64)
     out = np.zeros((n, n, n), dtype=np.float64)
-    for _ in range(n_blobs):
-        while True:
-            c = rng.uniform(-1.0, 1.0, 3)
-            if np.dot(c, c) <= 1.0:
-                break
-        c = n / 2.0 + centre_radius * c
-        s = rng.uniform(*sigma_range)
-        a = rng.uniform(0.5, 1.0)
+    for c, s, a in blob_params(n, n_blobs, seed, sigma_range, centre_radius):
         gx = np.exp(-(grid - c[0]) ** 2 / (2 * s * s))
         gy = np.exp(-(grid - c[1]) ** 2 / (2 * s * s))
         gz = np.exp(-(grid - c[2]) ** 2 / (2 * s * s))
@@ -120,6 +111,58 @@ def fourier_volume(density: np.ndarray, pad: int = 2, scale: float | None = None
     if scale is None:
         scale = 1.0 / float(n) ** 1.5
     vol *= scale
+    return ProjectorVolume(
+        real=np.ascontiguousarray(vol.real.astype(np.float32).ravel()),
+        imag=np.ascontiguousarray(vol.imag.astype(np.float32).ravel()),
+        mdlX=mdlX, mdlY=pad_size, mdlZ=pad_size, initY=init, initZ=init,
+        maxR=r_max, pad=float(pad))
+
+
+def blob_params(n: int, n_blobs: int = 20, seed: int = 0, sigma_range=(2.0, 6.0),
+                centre_radius: float | None = None):
+    """The (centre, sigma, amplitude) triples ``blob_map`` draws for a seed."""
+    rng = np.random.default_rng(seed)
+    if centre_radius is None:
+        centre_radius = 20.0 * n / 64.0
+    out = []
+    for _ in range(n_blobs):
+        while True:
+            c = rng.uniform(-1.0, 1.0, 3)
+            if np.dot(c, c) <= 1.0:
+                break
+        c = n / 2.0 + centre_radius * c
+        s = rng.uniform(*sigma_range)
+        a = rng.uniform(0.5, 1.0)
+        out.append((c, s, a))
+    return out
+
+
+def fourier_volume_analytic(n: int, r_max: int, pad: int = 2, seed: int = 0,
+                            n_blobs: int = 20) -> ProjectorVolume:
+    """Projector volume of the ``blob_map(n, seed=seed)`` map computed from
+    the closed-form Fourier transform of its Gaussians, only out to radius
+    ``r_max`` (so a coarse-pass volume of a large box costs no padded FFT):
+    F(k) = sum_b a_b (2 pi s_b^2)^{3/2} exp(-2 pi^2 s_b^2 |k|^2 / P^2)
+           exp(-2 pi i k . (c_b - n/2) / P),  P = pad * n,
+    scaled like ``fourier_volume``."""
+    p = pad * n
+    pad_size = 2 * (pad * r_max + 1) + 1
+    mdlX = pad_size // 2 + 1
+    init = -(pad_size // 2)
+    kyz = np.arange(pad_size, dtype=np.float64) + init
+    kx = np.arange(mdlX, dtype=np.float64)
+    kz3, ky3, kx3 = kyz[:, None, None], kyz[None, :, None], kx[None, None, :]
+    k2 = kx3 ** 2 + ky3 ** 2 + kz3 ** 2
+    vol = np.zeros((pad_size, pad_size, mdlX), dtype=np.complex128)
+    for c, s, a in blob_params(n, n_blobs, seed):
+        d = c - n / 2.0
+        amp = a * (2 * math.pi * s * s) ** 1.5 * np.exp(-2 * math.pi ** 2 * s * s * k2 / (p * p))
+        ph = np.exp(-2j * math.pi * d[0] * kx / p)[None, None, :] * \
+            np.exp(-2j * math.pi * d[1] * kyz / p)[None, :, None] * \
+            np.exp(-2j * math.pi * d[2] * kyz / p)[:, None, None]
+        vol += amp * ph
+    vol[k2 > (pad * r_max) ** 2] = 0
+    vol *= 1.0 / float(n) ** 1.5
     return ProjectorVolume(
         real=np.ascontiguousarray(vol.real.astype(np.float32).ravel()),
         imag=np.ascontiguousarray(vol.imag.astype(np.float32).ravel()),
@@ -426,30 +469,56 @@ class CoarseProblem:
 
 # name -> (map n, coarse imgY, healpix order, n_psi, r2_max, particles, snr, shift range)
 CONFIGS = {
-    "mini": dict(n=16, imgY=16, order=0, n_psi=4, r2=2, particles=2, snr=0.5, shift=1.0),
-    "c1": dict(n=64, imgY=32, order=1, n_psi=12, r2=5, particles=8, snr=0.1, shift=2.0),
-    "c2": dict(n=128, imgY=64, order=2, n_psi=24, r2=36, particles=512, snr=0.05, shift=2.0),
+    "mini": dict(n=16, imgY=16, order=0, n_psi=4, r2=2, particles=2, snr=0.5, shift=1.0,
+                 classes=1),
+    "c1": dict(n=64, imgY=32, order=1, n_psi=12, r2=5, particles=8, snr=0.1, shift=2.0,
+               classes=1),
+    "c2": dict(n=128, imgY=64, order=2, n_psi=24, r2=36, particles=512, snr=0.05, shift=2.0,
+               classes=1),
+    # batched 3D classification: per-particle translation radius 4..8 px
+    # around a random prior centre (49..197 shifts each)
+    "c4": dict(n=128, imgY=64, order=2, n_psi=24, r2=None, radius=(4, 8), particles=2048,
+               snr=0.05, shift=2.0, classes=8),
 }
 
 
 def make_problem(name: str = "c1", seed: int = 1234, particles: int | None = None,
-                 classes: int = 1, noise: bool = True, snr: float | None = None) -> CoarseProblem:
-    """Build one of the named workloads (``mini``, ``c1``, ``c2``).
+                 classes: int | None = None, noise: bool = True,
+                 snr: float | None = None) -> CoarseProblem:
+    """Build one of the named workloads (``mini``, ``c1``, ``c2``, ``c4``).
 
     ``c1``: 64^3 map, 32^2 coarse image (414 in-radius pixels), 576
     orientations (HEALPix order 1 x 12 psi), 21 shifts (r^2 <= 5), 8 particles
     at SNR 0.1.  ``c2``: 128^3 map, 64^2 coarse image (1636 in-radius
     pixels), 4608 orientations (order 2 x 24 psi), 113 shifts (r^2 <= 36),
-    512 particles at SNR 0.05.  ``mini``: a 16^3 map for fixtures.
+    512 particles at SNR 0.05.  ``c4``: 8 independent 128^3 classes, 2048
+    particles, each with its own translation set of radius 4-8 px around a
+    random prior centre (49-197 shifts).  ``mini``: a 16^3 map for fixtures.
     """
     c = CONFIGS[name]
-    vols = [fourier_volume(blob_map(c["n"], seed=seed + 7919 * k)) for k in range(classes)]
+    n_cls = c["classes"] if classes is None else classes
+    vols = [fourier_volume(blob_map(c["n"], seed=seed + 7919 * k)) for k in range(n_cls)]
     g = CoarseGeometry(imgY=c["imgY"], maxR=c["imgY"] // 2, pad=2.0)
     eul = orientation_set(c["order"], c["n_psi"])
-    tx, ty, _ = translation_set(c["r2"], c["n"])
     n = c["particles"] if particles is None else particles
+    rng = np.random.default_rng(seed + 2)
+    if c["r2"] is not None:
+        tx, ty, _ = translation_set(c["r2"], c["n"])
+        off = (np.arange(n + 1) * tx.size).astype(np.int32)
+        txs, tys = np.tile(tx, n), np.tile(ty, n)
+        centres = None
+    else:
+        centres = rng.uniform(-2.0, 2.0, (n, 2))
+        txl, tyl, counts = [], [], []
+        for i in range(n):
+            r = int(rng.integers(c["radius"][0], c["radius"][1] + 1))
+            tx, ty, _ = translation_set(r * r, c["n"], centre=tuple(centres[i]))
+            txl.append(tx)
+            tyl.append(ty)
+            counts.append(tx.size)
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        txs, tys = np.concatenate(txl), np.concatenate(tyl)
     parts = make_particles(vols, g, n, c["n"], c["snr"] if snr is None else snr, seed + 1,
-                           shift_range=c["shift"], noise=noise)
-    off = (np.arange(n + 1) * tx.size).astype(np.int32)
-    return CoarseProblem(name, vols, g, eul, off, np.tile(tx, n), np.tile(ty, n), parts,
-                         c["n"], meta=dict(seed=seed, **c))
+                           shift_range=c["shift"], noise=noise, centres=centres)
+    return CoarseProblem(name, vols, g, eul, off, txs, tys, parts, c["n"],
+                         meta=dict(seed=seed, **c))

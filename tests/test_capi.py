@@ -74,14 +74,11 @@ def test_error_paths_without_device(tmp_path):
     # zdim == 1 is a 2D reference: not on this path
     buf = (C.c_float * 16)()
     assert L.rc_projector_create(4, 4, 1, -2, 0, 2, 2.0, buf, buf, C.byref(h)) == 2
-    # cross-correlation mode is rejected before anything runs
+    # 3D data is rejected before anything runs
     k = lib.rc_projector_kernel()
     st = L.rc_run_diff2_kernel_coarse(C.byref(k), None, None, None, None, None, None, None,
-                                      None, 1.0, 1, 1, 1, None, 1, 0)
-    assert st == 2 and b"do_CC" in L.rc_last_error()
-    st = L.rc_run_diff2_kernel_coarse(C.byref(k), None, None, None, None, None, None, None,
                                       None, 1.0, 1, 1, 1, None, 0, 1)
-    assert st == 2
+    assert st == 2 and b"data_is_3D" in L.rc_last_error()
     # empty plan description
     d = lib.rc_batch_desc()
     p = C.c_void_p()

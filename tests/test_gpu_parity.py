@@ -106,7 +106,35 @@ def test_run_diff2_kernel_coarse_unfiltered(c1, lib, torch_cuda):
         _assert_close(d2.cpu().numpy(), ref[i].ravel() + 2.5, "fp32", f"unfiltered {i}")
 
 
-def test_run_diff2_kernel_coarse_rejects_cc(c1, lib, torch_cuda):
+def test_run_diff2_kernel_coarse_cc_mode(c1, lib, torch_cuda):
+    """do_CC: -sum corr Re(P conj Xs) / sqrt(sum corr |P|^2), written over
+    diff2s (diff2.cuh:691-809), against the float64 CC oracle."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob, _ = c1
+    pb = prob.particles
+    g = prob.geom
+    order, _ = synth.filter_order(g)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    proj = lib.Projector(prob.vols[0])
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    O = prob.orientation_num
+    eul = _t(torch, prob.eulers.ravel())
+    for i in (1, 6):
+        tx, ty = prob.trans_of(i)
+        T = tx.size
+        d2 = torch.full((O * T,), 123.0, dtype=torch.float32, device="cuda")
+        lib.run_diff2_kernel_coarse(pk, _t(torch, tx), _t(torch, ty), _t(torch, pb.corr[i][inv]),
+                                    _t(torch, pb.real[i][inv]), _t(torch, pb.imag[i][inv]), eul,
+                                    d2, O, T, g.image_size, do_CC=True)
+        torch.cuda.synchronize()
+        ref = oracle.cc_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers,
+                            tx, ty, pb.real[i][inv], pb.imag[i][inv], pb.corr[i][inv])
+        _assert_close(d2.cpu().numpy(), ref.ravel(), "fp32", f"cc {i}")
+
+
+def test_run_diff2_kernel_coarse_rejects_3d_data(c1, lib, torch_cuda):
     torch = torch_cuda
     prob, _ = c1
     g = prob.geom
@@ -114,8 +142,36 @@ def test_run_diff2_kernel_coarse_rejects_cc(c1, lib, torch_cuda):
     pk = proj.kernel(g.imgX, g.imgY, g.maxR)
     x = torch.zeros(4096, device="cuda")
     with pytest.raises(lib.RcError) as e:
-        lib.run_diff2_kernel_coarse(pk, x, x, x, x, x, x, x, 1, 1, 16, do_CC=True)
+        lib.run_diff2_kernel_coarse(pk, x, x, x, x, x, x, x, 1, 1, 16, data_is_3D=True)
     assert e.value.status == 2
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def test_batched_cc(c1, lib, torch_cuda, form):
+    """Batched cross-correlation scores; the per-particle minimum is the best
+    correlation."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob, _ = c1
+    pb = prob.particles
+    g = prob.geom
+    order, _ = synth.filter_order(g)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    projs = [lib.Projector(v) for v in prob.vols]
+    plan = lib.plan_for_problem(prob, projs, form, cc=True)
+    t = lib.problem_tensors(prob)
+    plan.execute(t)
+    torch.cuda.synchronize()
+    d = t["diff2"].cpu().numpy().astype(np.float64)
+    off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
+    bidx = t["best_index"].cpu().numpy()
+    for i in (0, 4):
+        tx, ty = prob.trans_of(i)
+        ref = oracle.cc_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers,
+                            tx, ty, pb.real[i][inv], pb.imag[i][inv], pb.corr[i][inv])
+        _assert_close(d[off[i]:off[i + 1]], ref.ravel(), form, f"batched cc {i}")
+        assert bidx[i] == oracle.argmin_flat(ref)
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])

@@ -298,3 +298,21 @@ def test_error_stats_follow_reference_metrics():
     assert st["count"] == 4 and st["count_rel"] == 3
     assert st["max_abs"] == 1.0
     assert abs(st["mean_rel"] - (0 + 1.0 + 0.25) / 3) < 1e-15
+
+
+def test_cc_score_against_diff2_expansion():
+    """Cross-correlation score (diff2.cuh:691-809) pinned by the expanded
+    diff2: sum w Re(P conj Xs) = sum w/2 |P|^2 + sum w/2 |X|^2 - diff2."""
+    v, g, x, y, npix, eul, tx, ty, re, im, w = _tiny_problem(9)
+    order, _ = synth.filter_order(g)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    xu, yu, reu, imu, wu = x[inv], y[inv], re[inv], im[inv], w[inv]
+    cc = oracle.cc_f64(v, g.maxR2_padded, g.imgX, g.imgY, g.maxR, eul, tx, ty, reu, imu, wu)
+    d2 = oracle.diff2_f64(v, g.maxR2_padded, eul, tx, ty, reu, imu, wu, xu, yu)
+    xn = np.sum(wu / 2.0 * (reu.astype(np.float64) ** 2 + imu.astype(np.float64) ** 2))
+    for o in range(eul.shape[0]):
+        P = oracle.project(v, g.maxR2_padded, eul[o], xu, yu)
+        pn = np.sum(wu / 2.0 * np.abs(P) ** 2)
+        want = -(pn + xn - d2[o]) / np.sqrt(2.0 * pn)
+        np.testing.assert_allclose(cc[o], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
