@@ -209,6 +209,32 @@ int rc_variant_count(int form);
 int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_gather);
 
 /* ---------------------------------------------------------------------- */
+/* Replay of the reference's coarse-particle dumps                         */
+/* ---------------------------------------------------------------------- */
+/* Host copy of one `<iter>/coarse_data<d>00_1.dat` file, read as
+ * CreateCoarseParamFromFile reads it (main.cu:153-203): int32 xdim, ydim,
+ * zdim, rmax, translation_num, orientation_num, image_size; then float32
+ * eulers[9*O], trans[2*T] (x block, y block), fimg[2*npix] (real block,
+ * imag block), corr[npix], diff2s[O*T].  (The reference harness passes
+ * eulers as the translation arrays, main.cu:234-235; trans_x/trans_y here
+ * are the file's own translation block.) */
+typedef struct {
+    int xdim, ydim, zdim, rmax;
+    int translation_num, orientation_num, image_size;
+    float *eulers;       /* [9*O]  */
+    float *trans_x;      /* [T]    */
+    float *trans_y;      /* [T]    */
+    float *real, *imag;  /* [npix] */
+    float *corr;         /* [npix] */
+    float *diff2s;       /* [O*T]  initial values */
+} rc_coarse_data;
+
+/* Missing or short file -> RC_ERR_IO; the arrays are malloc'd host memory
+ * released by rc_coarse_data_free. */
+int rc_coarse_data_load(const char *path, rc_coarse_data *out);
+void rc_coarse_data_free(rc_coarse_data *d);
+
+/* ---------------------------------------------------------------------- */
 /* Serving engine (no reference counterpart): continuous batching of       */
 /* coarse-search requests on one GPU                                       */
 /* ---------------------------------------------------------------------- */

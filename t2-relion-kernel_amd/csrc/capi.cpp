@@ -241,14 +241,18 @@ static TuneKey tune_key(const rc_batch_desc *d)
 
 namespace rc {
 
+// Variant used when the autotuner has not seen a shape: the fastest of the
+// compiled table at 512 x 4608 x 113 x 1636 on MI355X (pipelined 128 x 128
+// for FP32 and BF16x3, pipelined 64 x 256 with one row fragment per wave
+// for FP16).
 static int default_variant(int form, int P, int Tmax)
 {
     (void)P;
     (void)Tmax;
     switch (form) {
-    case kFP32: return 0;
-    case kBF16X3: return 2;
-    default: return 2;
+    case kFP32: return 4;
+    case kBF16X3: return 4;
+    default: return 7;
     }
 }
 
@@ -531,6 +535,62 @@ int rc_projector_create_from_file(const char *path, rc_projector **out)
     f.read(reinterpret_cast<char *>(im.data()), n * sizeof(float));
     RC_REQUIRE(f.good(), RC_ERR_IO, "projector file '%s' is truncated", path);
     return rc_projector_create(h[0], h[1], zdim, h[3], h[4], h[5], pad, re.data(), im.data(), out);
+}
+
+int rc_coarse_data_load(const char *path, rc_coarse_data *out)
+{
+    RC_REQUIRE(path && out, RC_ERR_INVALID, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    std::ifstream f(path, std::ios::binary);
+    RC_REQUIRE(f.is_open(), RC_ERR_IO, "cannot open coarse data file '%s'", path);
+    int32_t h[7];
+    f.read(reinterpret_cast<char *>(h), sizeof(h));
+    RC_REQUIRE(f.good(), RC_ERR_IO, "coarse data file '%s' has a truncated header", path);
+    RC_REQUIRE(h[4] > 0 && h[5] > 0 && h[6] > 0 && h[4] <= kMaxTrans && h[6] <= kMaxPix &&
+                   h[5] < (1 << 24),
+               RC_ERR_IO, "coarse data file '%s': implausible sizes T=%d O=%d npix=%d", path,
+               h[4], h[5], h[6]);
+    out->xdim = h[0];
+    out->ydim = h[1];
+    out->zdim = h[2];
+    out->rmax = h[3];
+    out->translation_num = h[4];
+    out->orientation_num = h[5];
+    out->image_size = h[6];
+    const size_t O = (size_t)h[5], T = (size_t)h[4], n = (size_t)h[6];
+    auto take = [&](size_t count) {
+        float *p = static_cast<float *>(malloc(std::max<size_t>(count, 1) * sizeof(float)));
+        if (p) f.read(reinterpret_cast<char *>(p), count * sizeof(float));
+        return p;
+    };
+    out->eulers = take(9 * O);
+    out->trans_x = take(T);
+    out->trans_y = take(T);
+    out->real = take(n);
+    out->imag = take(n);
+    out->corr = take(n);
+    out->diff2s = take(O * T);
+    const bool ok = f.good() && out->eulers && out->trans_x && out->trans_y && out->real &&
+                    out->imag && out->corr && out->diff2s;
+    if (!ok) {
+        rc_coarse_data_free(out);
+        set_error("coarse data file '%s' is truncated", path);
+        return RC_ERR_IO;
+    }
+    return RC_OK;
+}
+
+void rc_coarse_data_free(rc_coarse_data *d)
+{
+    if (!d) return;
+    free(d->eulers);
+    free(d->trans_x);
+    free(d->trans_y);
+    free(d->real);
+    free(d->imag);
+    free(d->corr);
+    free(d->diff2s);
+    memset(d, 0, sizeof(*d));
 }
 
 int rc_projector_destroy(rc_projector *p)

@@ -34,7 +34,8 @@ EXPORTS = [
     "rc_variant_count", "rc_variant_info", "rc_last_error", "rc_version", "rc_plan_timing",
     "rc_plan_timing_read", "rc_autotune", "rc_plan_capture", "rc_plan_replay",
     "rc_server_create", "rc_server_submit", "rc_server_wait", "rc_server_diff2",
-    "rc_server_release", "rc_server_stats", "rc_server_destroy",
+    "rc_server_release", "rc_server_stats", "rc_server_destroy", "rc_coarse_data_load",
+    "rc_coarse_data_free",
 ]
 
 

@@ -8,7 +8,9 @@ metrics (check.cuh:96-111).  Tolerances (relative to the float64 value):
   phase and accumulation against float64; the reference's own float32
   kernels sit at a mean of ~1e-7 against its float64 kernel);
 * BF16x3: max 1e-5, mean 2e-6, identical per-particle argmin;
-* FP16 (per-image scaled, one product): max 5e-3, mean 5e-4.
+* FP16 (per-image scaled, one product): max 5e-3, mean 5e-4;
+* cross-correlation scores (which cross zero): max |error| <= 1e-5 (FP32) or
+  3e-5 (BF16x3) of the largest |score|.
 """
 import numpy as np
 import pytest
@@ -41,6 +43,16 @@ def _t(torch, a, dt=None):
     if dt is not None:
         x = x.to(dt)
     return x.cuda()
+
+
+def _assert_close_scaled(got, ref, tol, what):
+    """Cross-correlation scores cross zero, so their error is measured
+    against the score's scale: max |got - ref| <= tol * max |ref|."""
+    got = np.asarray(got, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"{what} scaled_max_err={err:.3e}")
+    assert err <= tol, (what, err)
 
 
 def _assert_close(got, ref, form, what):
@@ -131,7 +143,7 @@ def test_run_diff2_kernel_coarse_cc_mode(c1, lib, torch_cuda):
         torch.cuda.synchronize()
         ref = oracle.cc_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers,
                             tx, ty, pb.real[i][inv], pb.imag[i][inv], pb.corr[i][inv])
-        _assert_close(d2.cpu().numpy(), ref.ravel(), "fp32", f"cc {i}")
+        _assert_close_scaled(d2.cpu().numpy(), ref.ravel(), 1e-5, f"cc {i}")
 
 
 def test_run_diff2_kernel_coarse_rejects_3d_data(c1, lib, torch_cuda):
@@ -170,7 +182,8 @@ def test_batched_cc(c1, lib, torch_cuda, form):
         tx, ty = prob.trans_of(i)
         ref = oracle.cc_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers,
                             tx, ty, pb.real[i][inv], pb.imag[i][inv], pb.corr[i][inv])
-        _assert_close(d[off[i]:off[i + 1]], ref.ravel(), form, f"batched cc {i}")
+        _assert_close_scaled(d[off[i]:off[i + 1]], ref.ravel(), 1e-5 if form == "fp32" else 3e-5,
+                             f"batched cc {i}")
         assert bidx[i] == oracle.argmin_flat(ref)
 
 
