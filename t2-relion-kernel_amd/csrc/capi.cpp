@@ -245,15 +245,16 @@ namespace rc {
 // compiled table at 512 x 4608 x 113 x 1636 on MI355X (pipelined 128 x 128
 // for FP32 and BF16x3, pipelined 64 x 256 with one row fragment per wave
 // for FP16).
-static int default_variant(int form, int P, int Tmax)
+// Measured on MI355X (512 particles x 113 translations x 4608 orientations x
+// 1636 pixels): the wide staggered 256 x 256 kernel is the fastest of every
+// form once its tiles fill the device; smaller searches use the pipelined
+// 128 x 128 kernel (FP16: 64 x 256), whose finer tiles waste less padding.
+static int default_variant(int form, int n_groups, int n_cols)
 {
-    (void)P;
-    (void)Tmax;
-    switch (form) {
-    case kFP32: return 4;
-    case kBF16X3: return 4;
-    default: return 7;
-    }
+    constexpr int kWide = 14;   // first table entry past 10: wide, 256 x 256, staggered
+    const long long tiles = (long long)((n_groups * 16 + 255) / 256) * ((n_cols + 255) / 256);
+    if (tiles >= 512) return kWide;
+    return form == kFP16 ? 7 : 4;
 }
 
 int validate_batch(const rc_batch_desc *d, bool need_ptrs)
@@ -320,12 +321,10 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
     p->kpad = (int)align_up((size_t)d->npix, kChunk);
     p->nkc = p->kpad / kChunk;
     p->trans_offset.assign(d->host_trans_offset, d->host_trans_offset + d->n_particles + 1);
-    int tmax = 0;
     tab.groups.clear();
     tab.group_begin.assign(p->P + 1, 0);
     for (int i = 0; i < p->P; ++i) {
         const int T = p->trans_offset[i + 1] - p->trans_offset[i];
-        tmax = std::max(tmax, T);
         tab.group_begin[i] = (int)tab.groups.size();
         for (int t0 = 0; t0 < T; t0 += 16) tab.groups.push_back(make_int2(i, t0));
     }
@@ -335,7 +334,8 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
     if (vi < 0) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(d));
-        vi = it != g_tuned.end() ? it->second : default_variant(p->form, p->P, tmax);
+        vi = it != g_tuned.end() ? it->second
+                                  : default_variant(p->form, p->n_groups, p->C * p->O);
     }
     const VariantInfo *info = variant_info(p->form, false, vi);
     RC_REQUIRE(info, RC_ERR_INVALID, "variant %d does not exist for form %d", vi, d->form);
@@ -411,8 +411,12 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
     RC_HIP(launch_prep(p->P, p->npix, p->full_npix, p->kpad, d->d_real, d->d_imag, d->d_corr,
                        d->d_coor_xy, k0.imgX, k0.imgY, k0.maxR, p->xw, p->coords, p->xnorm,
                        p->half_corr, p->ascale, s, d->do_cc != 0));
-    RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
-                          p->npix, p->kpad, p->proj, p->pabs2, s));
+    if (p->info.wide)
+        RC_HIP(launch_project_wide(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O,
+                                   p->coords, p->npix, p->kpad, p->proj, p->pabs2, s));
+    else
+        RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
+                              p->npix, p->kpad, p->proj, p->pabs2, s));
     RC_HIP(launch_pnorm(p->P, p->C * p->O, p->kpad, p->half_corr, p->pabs2, p->pnorm, s));
     CrossArgs a{};
     a.groups = p->groups;

@@ -783,11 +783,6 @@ static VariantInfo info_pipe()
                        PERSIST};
 }
 
-struct VariantEntry {
-    VariantInfo info;
-    hipError_t (*launch)(const CrossArgs &, hipStream_t);
-};
-
 #define RC_VARIANT(F, WM, WN, FM, FN, FU) \
     VariantEntry{info_one<F, WM, WN, FM, FN, FU>(), &launch_one<F, WM, WN, FM, FN, FU>}
 #define RC_PIPE(F, WM, WN, FM, FN, PE) \
@@ -834,27 +829,37 @@ static const VariantEntry *table(int form, bool fused, int *n)
     }
 }
 
-int variant_count(int form, bool fused)
+// Non-fused indices continue past the table with the wide kernels.
+static const VariantEntry *entry(int form, bool fused, int idx)
 {
     int n = 0;
+    const VariantEntry *t = table(form, fused, &n);
+    if (!t || idx < 0) return nullptr;
+    if (idx < n) return &t[idx];
+    if (fused) return nullptr;
+    int nw = 0;
+    const VariantEntry *w = wide_table(form, &nw);
+    return idx - n < nw ? &w[idx - n] : nullptr;
+}
+
+int variant_count(int form, bool fused)
+{
+    int n = 0, nw = 0;
     table(form, fused, &n);
-    return n;
+    if (!fused && n) wide_table(form, &nw);
+    return n + nw;
 }
 
 const VariantInfo *variant_info(int form, bool fused, int idx)
 {
-    int n = 0;
-    const VariantEntry *t = table(form, fused, &n);
-    if (!t || idx < 0 || idx >= n) return nullptr;
-    return &t[idx].info;
+    const VariantEntry *e = entry(form, fused, idx);
+    return e ? &e->info : nullptr;
 }
 
 hipError_t launch_cross(int form, bool fused, int idx, const CrossArgs &a, hipStream_t s)
 {
-    int n = 0;
-    const VariantEntry *t = table(form, fused, &n);
-    if (!t || idx < 0 || idx >= n) return hipErrorInvalidValue;
-    return t[idx].launch(a, s);
+    const VariantEntry *e = entry(form, fused, idx);
+    return e ? e->launch(a, s) : hipErrorInvalidValue;
 }
 
 }  // namespace rc

@@ -1,0 +1,632 @@
+// "Wide" cross-term kernel: both MFMA operands staged in LDS, the
+// translation operand built once per block by all its threads.
+//
+// The pixel axis is walked in 16-pixel chunks whose real and imaginary parts
+// are interleaved along the MFMA k axis (k = 2 * pixel + component), so one
+// 16x16x32 BF16/FP16 MFMA (or eight 16x16x4 FP32 MFMAs) consumes a whole
+// chunk: Re(P conj A) = sum_k P_k A_k.  Per chunk:
+//   * B (projections): LDS-DMA from the pre-encoded projection buffer;
+//   * A (translated, weighted images): each thread produces 8 consecutive
+//     pixels of one row -- phase factor from an exactly reduced argument,
+//     times corr*X, encoded -- and stores whole 16-byte segments to LDS;
+//   * every wave reads its row and column fragments from LDS and issues the
+//     MFMAs.  A for chunk k+1 and the B copy of chunk k+1 are produced while
+//     the MFMAs of chunk k run; both buffers are double-buffered, one
+//     barrier per chunk.
+// Row/column r of a buffer holds KB bytes (128: FP32/BF16X3, 64: FP16) as
+// 16-byte segments; segment s sits at position s ^ ((r >> FSH) & (SEGS-1)),
+// which spreads the 16 rows read by one k-group of a wave over all banks.
+//
+// Operand encodings of one row/column for one chunk (pixel p, component c):
+//   BF16X3: hi[32] bf16 at k = 2p + c (bytes 0..63), lo[32] (bytes 64..127)
+//   FP16:   [32] f16 at k = 2p + c, scaled by a power of two
+//   FP32:   value v = 2p + c at float (v % 4) * 8 + v / 4, so lane group q
+//           reads floats 8q .. 8q+7 and MFMA m pairs k-slot q with v = 4m+q.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+
+#include "device_common.hpp"
+
+namespace rc {
+
+constexpr double kInv2PiW = 0.15915494309189533577;
+
+template <int FORM>
+struct WideEnc {
+    static constexpr int kBytes = FORM == kFP16 ? 64 : 128;
+    static constexpr int kSegs = kBytes / 16;
+    static constexpr int kShift = kSegs == 8 ? 1 : 2;   // rows per 256-byte bank row
+};
+
+template <int FORM>
+__device__ __forceinline__ int wide_swz(int r, int s)
+{
+    return s ^ ((r >> WideEnc<FORM>::kShift) & (WideEnc<FORM>::kSegs - 1));
+}
+
+// Encode one complex value (pixel p in [0,16), parts re/im) into an unswizzled
+// row image `row` of WideEnc<FORM>::kBytes bytes (used by the projection
+// precompute, one value per thread).
+template <int FORM>
+__device__ __forceinline__ void wide_store_value(unsigned char *row, int p, float re, float im,
+                                                 float scale)
+{
+    if constexpr (FORM == kBF16X3) {
+        __bf16 *b = reinterpret_cast<__bf16 *>(row);
+        __bf16 h, l;
+        split_bf16(re, h, l);
+        b[2 * p] = h;
+        b[32 + 2 * p] = l;
+        split_bf16(im, h, l);
+        b[2 * p + 1] = h;
+        b[32 + 2 * p + 1] = l;
+    } else if constexpr (FORM == kFP16) {
+        _Float16 *h = reinterpret_cast<_Float16 *>(row);
+        h[2 * p] = (_Float16)(re * scale);
+        h[2 * p + 1] = (_Float16)(im * scale);
+    } else {
+        float *f = reinterpret_cast<float *>(row);
+        const int v0 = 2 * p, v1 = 2 * p + 1;
+        f[(v0 & 3) * 8 + (v0 >> 2)] = re;
+        f[(v1 & 3) * 8 + (v1 >> 2)] = im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// projection precompute in the wide encoding: block = 16 orientations x one
+// 16-pixel chunk
+// ---------------------------------------------------------------------------
+template <int FORM>
+__global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__restrict__ vols,
+                                                           const float *__restrict__ bscale,
+                                                           const float *__restrict__ eulers,
+                                                           int O, const float2 *__restrict__ coords,
+                                                           int npix, int kpad,
+                                                           unsigned char *__restrict__ proj,
+                                                           float *__restrict__ pabs2)
+{
+    constexpr int KB = WideEnc<FORM>::kBytes;
+    __shared__ __attribute__((aligned(16))) unsigned char s_out[16 * KB];
+    const int nkc = kpad / 16;
+    const int kc = blockIdx.x, cls = blockIdx.z;
+    const int ol = threadIdx.x >> 4, pl = threadIdx.x & 15;
+    const int o = blockIdx.y * 16 + ol;
+    const int p = kc * 16 + pl;
+    const ProjView v = vols[cls];
+    float2 val = make_float2(0.f, 0.f);
+    if (o < O && p < npix) {
+        const float *e = eulers + 9 * (size_t)o;
+        const float2 xy = coords[p];
+        val = project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+    }
+    const float sc = (FORM == kFP16) ? bscale[cls] : 1.f;
+    wide_store_value<FORM>(s_out + ol * KB, pl, val.x, val.y, sc);
+    if (o < O) pabs2[((size_t)cls * O + o) * kpad + p] = val.x * val.x + val.y * val.y;
+    __syncthreads();
+    constexpr int units = 16 * KB / 16;
+    for (int u = threadIdx.x; u < units; u += 256) {
+        const int oo = blockIdx.y * 16 + u / (KB / 16);
+        if (oo < O) {
+            const int seg = u % (KB / 16);
+            const uint4 d = reinterpret_cast<const uint4 *>(s_out)[u];
+            reinterpret_cast<uint4 *>(proj + (((size_t)cls * nkc + kc) * O + oo) * KB)[seg] = d;
+        }
+    }
+}
+
+hipError_t launch_project_wide(int form, const ProjView *vols, const float *bscale, int n_classes,
+                               const float *eulers, int O, const float2 *coords, int npix,
+                               int kpad, unsigned char *proj, float *pabs2, hipStream_t s)
+{
+    dim3 grid(kpad / 16, (O + 15) / 16, n_classes);
+    switch (form) {
+    case kFP32:
+        hipLaunchKernelGGL(project_wide_kernel<kFP32>, grid, dim3(256), 0, s, vols, bscale,
+                           eulers, O, coords, npix, kpad, proj, pabs2);
+        break;
+    case kBF16X3:
+        hipLaunchKernelGGL(project_wide_kernel<kBF16X3>, grid, dim3(256), 0, s, vols, bscale,
+                           eulers, O, coords, npix, kpad, proj, pabs2);
+        break;
+    default:
+        hipLaunchKernelGGL(project_wide_kernel<kFP16>, grid, dim3(256), 0, s, vols, bscale,
+                           eulers, O, coords, npix, kpad, proj, pabs2);
+        break;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// A generation: 8 consecutive pixels of one row, written as whole segments
+// ---------------------------------------------------------------------------
+// Rows past a particle's last translation (and rows of absent groups) keep
+// finite values -- zero shift, the particle's own image -- and need no
+// masking: row t of the product depends on row t of A only, and the
+// epilogue never stores or ranks such rows.
+struct ARow {
+    const float2 *xw;      // this row's particle: corr * X, padded to kpad
+    double txr, tyr;       // translation in revolutions per pixel
+    float scale;           // FP16 form: power-of-two image scale
+};
+
+__device__ __forceinline__ float2 a_value(const ARow &r, float x, float y, float2 w)
+{
+    float s, c;
+    phase_factor(x, y, r.txr, r.tyr, s, c);
+    return make_float2(c * w.x - s * w.y, c * w.y + s * w.x);
+}
+
+// Value j (0..7) of the thread's 8 pixels of a chunk.
+__device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[4],
+                                          const float4 (&cxy)[4], int j)
+{
+    const float4 c = cxy[j >> 1], w = xw[j >> 1];
+    const float x = (j & 1) ? c.z : c.x, y = (j & 1) ? c.w : c.y;
+    const float2 ww = (j & 1) ? make_float2(w.z, w.w) : make_float2(w.x, w.y);
+    return a_value(ar, x, y, ww);
+}
+
+// Encodes the thread's 8 values (half h of row r) into LDS row buffer `dst`.
+template <int FORM>
+__device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[8], int h, int r,
+                                        unsigned char *dst)
+{
+    if constexpr (FORM == kBF16X3) {
+        // segments 2h, 2h+1 (hi) and 4+2h, 5+2h (lo), 4 pixels (re, im) each
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            uint32_t hi[4], lo[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 z = v[4 * half + i];
+                __bf16 hr, lr, hm, lm;
+                split_bf16(z.x, hr, lr);
+                split_bf16(z.y, hm, lm);
+                hi[i] = (uint32_t)__builtin_bit_cast(uint16_t, hr) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, hm) << 16);
+                lo[i] = (uint32_t)__builtin_bit_cast(uint16_t, lr) |
+                        ((uint32_t)__builtin_bit_cast(uint16_t, lm) << 16);
+            }
+            const int sh = 2 * h + half, sl = 4 + 2 * h + half;
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, sh) * 16) =
+                make_uint4(hi[0], hi[1], hi[2], hi[3]);
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, sl) * 16) =
+                make_uint4(lo[0], lo[1], lo[2], lo[3]);
+        }
+    } else if constexpr (FORM == kFP16) {
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            uint32_t w[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 z = v[4 * half + i];
+                const _Float16 re = (_Float16)(z.x * ar.scale), im = (_Float16)(z.y * ar.scale);
+                w[i] = (uint32_t)__builtin_bit_cast(uint16_t, re) |
+                       ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
+            }
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, 2 * h + half) * 16) =
+                make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+        // segment 2q + h holds component q % 2 of pixels 2i + q / 2, i = 0..3
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            float f[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float2 z = v[2 * i + (q >> 1)];
+                f[i] = (q & 1) ? z.y : z.x;
+            }
+            *reinterpret_cast<float4 *>(dst + wide_swz<FORM>(r, 2 * q + h) * 16) =
+                make_float4(f[0], f[1], f[2], f[3]);
+        }
+    }
+}
+
+// Builds the thread's part (half h of the row) of a chunk into LDS row
+// buffer `dst` (row index r for the swizzle).
+template <int FORM>
+__device__ __forceinline__ void a_build(const ARow &ar, const float4 (&xw)[4],
+                                        const float4 (&cxy)[4], int h, int r, unsigned char *dst)
+{
+    float2 v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = a_pixel(ar, xw, cxy, j);
+    a_store<FORM>(ar, v, h, r, dst);
+}
+
+// Operand registers of one 16-row (or 16-column) slice for one chunk.
+template <int FORM>
+struct WideFrag {
+    bf16x8 hi, lo;
+};
+template <>
+struct WideFrag<kFP16> {
+    halfx8 v;
+};
+template <>
+struct WideFrag<kFP32> {
+    float v[8];
+};
+
+// Lane (r16, q) reads row `r` (base `row`) of a staged buffer.
+template <int FORM>
+__device__ __forceinline__ void wide_load(WideFrag<FORM> &f, const unsigned char *row, int r,
+                                          int q)
+{
+    if constexpr (FORM == kBF16X3) {
+        f.hi = *reinterpret_cast<const bf16x8 *>(row + wide_swz<FORM>(r, q) * 16);
+        f.lo = *reinterpret_cast<const bf16x8 *>(row + wide_swz<FORM>(r, 4 + q) * 16);
+    } else if constexpr (FORM == kFP16) {
+        f.v = *reinterpret_cast<const halfx8 *>(row + wide_swz<FORM>(r, q) * 16);
+    } else {
+        const float4 x0 = *reinterpret_cast<const float4 *>(row + wide_swz<FORM>(r, 2 * q) * 16);
+        const float4 x1 =
+            *reinterpret_cast<const float4 *>(row + wide_swz<FORM>(r, 2 * q + 1) * 16);
+        f.v[0] = x0.x;
+        f.v[1] = x0.y;
+        f.v[2] = x0.z;
+        f.v[3] = x0.w;
+        f.v[4] = x1.x;
+        f.v[5] = x1.y;
+        f.v[6] = x1.z;
+        f.v[7] = x1.w;
+    }
+}
+
+// One column slice against all FM row slices.  The accumulators of one
+// column slice are independent, so consecutive MFMAs never wait on each
+// other's results.
+template <int FORM, int FM>
+__device__ __forceinline__ void wide_mfma(floatx4 (&acc)[FM], const WideFrag<FORM> (&a)[FM],
+                                          const WideFrag<FORM> &b)
+{
+    if constexpr (FORM == kBF16X3) {
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa)
+            acc[fa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fa].lo, b.hi, acc[fa], 0, 0, 0);
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa)
+            acc[fa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fa].hi, b.lo, acc[fa], 0, 0, 0);
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa)
+            acc[fa] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[fa].hi, b.hi, acc[fa], 0, 0, 0);
+    } else if constexpr (FORM == kFP16) {
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa)
+            acc[fa] = __builtin_amdgcn_mfma_f32_16x16x32_f16(a[fa].v, b.v, acc[fa], 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int fa = 0; fa < FM; ++fa)
+                acc[fa] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[fa].v[m], b.v[m], acc[fa], 0, 0,
+                                                               0);
+    }
+}
+
+typedef __attribute__((address_space(3))) void lds_void_w;
+typedef __attribute__((address_space(1))) const void gbl_void_w;
+
+template <int FORM, int WM, int WN, int FM, int FN>
+struct WideCfg {
+    static constexpr int kThreads = 64 * WM * WN;
+    static constexpr int kTileM = WM * FM * 16;
+    static constexpr int kTileN = WN * FN * 16;
+    static constexpr int kKB = WideEnc<FORM>::kBytes;
+    static constexpr int kBufA = kTileM * kKB;
+    static constexpr int kBufB = kTileN * kKB;
+    static constexpr int kLds = 2 * (kBufA + kBufB);
+};
+
+// STAGGER 0: every wave runs [MFMAs of chunk k, build A of chunk k+1].
+// STAGGER 1/2: the waves with bit 2 (1) or bit 0 (2) of their index set run
+// [build A of chunk k+1, MFMAs of chunk k] instead, so that the two waves a
+// SIMD holds keep its matrix core and its vector ALU busy at the same time.
+// STAGGER 4: each column slice's MFMAs share a fenced scheduling segment
+// with one pixel of the next chunk's A build.
+template <int FORM, int WM, int WN, int FM, int FN, int STAGGER>
+__global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArgs a)
+{
+    using Cfg = WideCfg<FORM, WM, WN, FM, FN>;
+    constexpr int NT = Cfg::kThreads;
+    constexpr int TM = Cfg::kTileM;
+    constexpr int TN = Cfg::kTileN;
+    constexpr int KB = Cfg::kKB;
+    constexpr int SEGS = WideEnc<FORM>::kSegs;
+    static_assert(2 * TM == NT, "each thread builds half a row of a chunk");
+    constexpr int UNITS = TN * SEGS;
+    static_assert(UNITS % NT == 0, "B tile must split evenly into 16-byte loads");
+    constexpr int LPT = UNITS / NT;
+
+    __shared__ __attribute__((aligned(16))) unsigned char sA[2][Cfg::kBufA];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[2][Cfg::kBufB];
+
+    const int n_ntiles = a.n_classes * a.ntiles_per_class;
+    const int total = a.n_mtiles * n_ntiles;
+    const int per_xcd = (total + 7) >> 3;
+    const int L = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (L >= total) return;
+    const int ntile = L / a.n_mtiles;
+    const int mtile = L - ntile * a.n_mtiles;
+    const int cls = ntile / a.ntiles_per_class;
+    const int o0 = (ntile - cls * a.ntiles_per_class) * TN;
+    const int O = a.orientation_num;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+    const int r16 = lane & 15, q = lane >> 4;
+    const int nkc = a.kpad / 16;
+
+    // ---- this thread's A row (row gr of the tile, pixels 8 gh .. 8 gh + 7 of a chunk)
+    const int gr = tid >> 1, gh = tid & 1;
+    ARow ar;
+    {
+        const int g = mtile * (TM / 16) + (gr >> 4);
+        int part = 0, t0 = 0, T = 0, base = 0;
+        if (g < a.n_groups) {
+            if (a.groups) {
+                const int2 gi = a.groups[g];
+                part = gi.x;
+                t0 = gi.y;
+            } else {
+                t0 = 16 * g;
+            }
+            if (a.trans_offset) {
+                base = a.trans_offset[part];
+                T = a.trans_offset[part + 1] - base;
+            } else {
+                T = a.n_trans;
+            }
+        }
+        const int t = t0 + (gr & 15);
+        const bool ok = t < T;
+        ar.txr = ok ? (double)a.trans_x[base + t] * kInv2PiW : 0.0;
+        ar.tyr = ok ? (double)a.trans_y[base + t] * kInv2PiW : 0.0;
+        ar.scale = (FORM == kFP16) ? a.ascale[part] : 1.f;
+        ar.xw = a.xw + (size_t)part * a.kpad + 8 * gh;
+    }
+    const float2 *c_row = a.coords + 8 * gh;
+
+    floatx4 acc[FN][FM];
+#pragma unroll
+    for (int fb = 0; fb < FN; ++fb)
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa) acc[fb][fa] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // projections: [class][chunk][orientation][KB]; the chunk of a column
+    // tile is one contiguous run of TN * KB bytes
+    const unsigned char *cls_proj = a.proj + (size_t)cls * nkc * O * KB;
+    auto issue_B = [&](int kc, int buf) {
+        const unsigned char *chunk = cls_proj + (size_t)kc * O * KB;
+#pragma unroll
+        for (int it = 0; it < LPT; ++it) {
+            const int u = it * NT + tid;
+            const int col = u / SEGS, pos = u % SEGS;
+            const int seg = wide_swz<FORM>(col, pos);   // the swizzle is an involution
+            const int o = min(o0 + col, O - 1);
+            const unsigned char *src = chunk + (size_t)o * KB + seg * 16;
+            unsigned char *dst = &sB[buf][(it * NT + wave * 64) * 16];
+            __builtin_amdgcn_global_load_lds((gbl_void_w *)src, (lds_void_w *)dst, 16, 0, 0);
+        }
+    };
+    auto load_src = [&](int kc, float4 (&xw)[4], float4 (&cxy)[4]) {
+        const float4 *xp = reinterpret_cast<const float4 *>(ar.xw + kc * 16);
+        const float4 *cp = reinterpret_cast<const float4 *>(c_row + kc * 16);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            xw[u] = xp[u];
+            cxy[u] = cp[u];
+        }
+    };
+
+    const int ra0 = wm * FM * 16, rb0 = wn * FN * 16;
+    auto mfma_chunk = [&](int buf) {
+        const unsigned char *pa = sA[buf];
+        const unsigned char *pb = sB[buf];
+        WideFrag<FORM> fa_reg[FM];
+#pragma unroll
+        for (int fa = 0; fa < FM; ++fa) {
+            const int ra = ra0 + fa * 16 + r16;
+            wide_load<FORM>(fa_reg[fa], pa + ra * KB, ra, q);
+        }
+#pragma unroll
+        for (int fb = 0; fb < FN; ++fb) {
+            const int rb = rb0 + fb * 16 + r16;
+            WideFrag<FORM> fb_reg;
+            wide_load<FORM>(fb_reg, pb + rb * KB, rb, q);
+            wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg);
+        }
+    };
+
+    // chunks past the projected pixels hold zeros: skip them
+    const int nrun = (a.npix + 15) / 16;
+    const bool lead = STAGGER == 0 || STAGGER == 4 ||
+                      (((STAGGER == 1 ? (wave >> 2) : wave) & 1) == 0);
+    float4 xw[4], cxy[4];
+    issue_B(0, 0);
+    load_src(0, xw, cxy);
+    a_build<FORM>(ar, xw, cxy, gh, gr, &sA[0][gr * KB]);
+    if (!lead || STAGGER == 4) load_src(min(1, nrun - 1), xw, cxy);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+
+    for (int kc = 0; kc < nrun; ++kc) {
+        const int buf = kc & 1;
+        // the last step fetches its own chunk again into the idle buffers,
+        // which keeps each path through the loop body one straight block
+        const int kn = min(kc + 1, nrun - 1);
+        issue_B(kn, buf ^ 1);
+        if (STAGGER == 4) {
+            // one column slice of MFMAs and one pixel of the next chunk's A
+            // per fenced segment; the image data for the next build were
+            // loaded at the end of the previous step
+            const unsigned char *pa = sA[buf];
+            const unsigned char *pb = sB[buf];
+            WideFrag<FORM> fa_reg[FM];
+#pragma unroll
+            for (int fa = 0; fa < FM; ++fa) {
+                const int ra = ra0 + fa * 16 + r16;
+                wide_load<FORM>(fa_reg[fa], pa + ra * KB, ra, q);
+            }
+            WideFrag<FORM> fb_reg[2];
+            wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
+            float2 v[8];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int fb = 0; fb < FN; ++fb) {
+                if (fb + 1 < FN) {
+                    const int rb = rb0 + (fb + 1) * 16 + r16;
+                    wide_load<FORM>(fb_reg[(fb + 1) & 1], pb + rb * KB, rb, q);
+                }
+                wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg[fb & 1]);
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    if ((j * FN) / 8 == fb) v[j] = a_pixel(ar, xw, cxy, j);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            a_store<FORM>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
+            load_src(min(kn + 1, nrun - 1), xw, cxy);
+        } else if (lead) {
+            load_src(kn, xw, cxy);
+            mfma_chunk(buf);
+            a_build<FORM>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+        } else {
+            a_build<FORM>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+            load_src(min(kn + 1, nrun - 1), xw, cxy);
+            mfma_chunk(buf);
+        }
+        if (STAGGER == 4)
+            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");   // the chunk's B copies
+        else
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+    }
+
+    // ---- epilogue: float64 assembly, stores, (min, argmin) partials
+    const int C = a.n_classes;
+#pragma unroll
+    for (int fa = 0; fa < FM; ++fa) {
+        const int g = mtile * (TM / 16) + wm * FM + fa;
+        int part = 0, t0 = 0, T = 0, base = 0;
+        if (g < a.n_groups) {
+            if (a.groups) {
+                const int2 gi = a.groups[g];
+                part = gi.x;
+                t0 = gi.y;
+            } else {
+                t0 = 16 * g;
+            }
+            if (a.trans_offset) {
+                base = a.trans_offset[part];
+                T = a.trans_offset[part + 1] - base;
+            } else {
+                T = a.n_trans;
+            }
+        }
+        const size_t out_base = (size_t)C * O * base + (size_t)cls * O * T;
+        const double xn = a.xnorm[part];
+        const float inv = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
+        double best = DBL_MAX;
+        long long best_k = 0x7fffffffffffffffLL;
+#pragma unroll
+        for (int fb = 0; fb < FN; ++fb) {
+            const int o = o0 + rb0 + fb * 16 + r16;
+            if (o >= O) continue;
+            const float pn = a.pnorm[((size_t)part * C + cls) * O + o];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int t = t0 + 4 * q + v;
+                if (t >= T) continue;
+                const size_t k = out_base + (size_t)o * T + t;
+                const float cval = acc[fb][fa][v] * inv;
+                double d;
+                if (a.cc) {
+                    d = -(double)cval / sqrt(2.0 * (double)pn);
+                } else {
+                    d = (double)pn + xn - (double)cval;
+                    if (a.diff2_init) d += (double)a.diff2_init[k];
+                }
+                a.diff2[k] = (float)d;
+                const long long flat = (long long)(cls * O + o) * T + t;
+                if (d < best || (d == best && flat < best_k)) {
+                    best = d;
+                    best_k = flat;
+                }
+            }
+        }
+        if (a.part_min) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off, 64);
+                const long long ok = __shfl_xor(best_k, off, 64);
+                if (ob < best || (ob == best && ok < best_k)) {
+                    best = ob;
+                    best_k = ok;
+                }
+            }
+            if (lane == 0 && g < a.n_groups) {
+                const size_t u = ((size_t)g * n_ntiles + ntile) * WN + wn;
+                a.part_min[u] = best;
+                a.part_idx[u] = best_k;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// wide variant table
+// ---------------------------------------------------------------------------
+template <int FORM, int WM, int WN, int FM, int FN, int ST>
+static hipError_t launch_wide_one(const CrossArgs &a, hipStream_t s)
+{
+    using Cfg = WideCfg<FORM, WM, WN, FM, FN>;
+    const int total = a.n_mtiles * a.n_classes * a.ntiles_per_class;
+    const int grid = ((total + 7) / 8) * 8;
+    hipLaunchKernelGGL((cross_kernel_wide<FORM, WM, WN, FM, FN, ST>), dim3(grid),
+                       dim3(Cfg::kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int FORM, int WM, int WN, int FM, int FN, int ST>
+static VariantEntry wide_one()
+{
+    using Cfg = WideCfg<FORM, WM, WN, FM, FN>;
+    return VariantEntry{VariantInfo{FORM, Cfg::kTileM, Cfg::kTileN, Cfg::kThreads, WN, false,
+                                    Cfg::kLds, false, true},
+                        &launch_wide_one<FORM, WM, WN, FM, FN, ST>};
+}
+
+#define RC_WIDE_TABLE(F)                                                              \
+    static const VariantEntry k_wide_##F[] = {                                         \
+        wide_one<F, 4, 2, 4, 8, 0>(), /* 256 x 256, 512 threads */                     \
+        wide_one<F, 2, 2, 4, 8, 0>(), /* 128 x 256, 256 threads */                     \
+        wide_one<F, 4, 2, 4, 4, 0>(), /* 256 x 128, 512 threads */                     \
+        wide_one<F, 2, 2, 4, 4, 0>(), /* 128 x 128, 256 threads */                     \
+        wide_one<F, 4, 2, 4, 8, 1>(), /* 256 x 256, staggered by wave bit 2 */         \
+        wide_one<F, 4, 2, 4, 8, 2>(), /* 256 x 256, staggered by wave bit 0 */         \
+        wide_one<F, 4, 2, 4, 4, 1>(), /* 256 x 128, staggered by wave bit 2 */         \
+        wide_one<F, 4, 2, 4, 8, 4>(), /* 256 x 256, fenced segments */                 \
+        wide_one<F, 4, 2, 4, 4, 4>(), /* 256 x 128, fenced segments */                 \
+    };
+
+RC_WIDE_TABLE(kFP32)
+RC_WIDE_TABLE(kBF16X3)
+RC_WIDE_TABLE(kFP16)
+
+const VariantEntry *wide_table(int form, int *n)
+{
+    *n = (int)(sizeof(k_wide_kFP32) / sizeof(k_wide_kFP32[0]));
+    switch (form) {
+    case kFP32: return k_wide_kFP32;
+    case kBF16X3: return k_wide_kBF16X3;
+    case kFP16: return k_wide_kFP16;
+    default:
+        *n = 0;
+        return nullptr;
+    }
+}
+
+}  // namespace rc

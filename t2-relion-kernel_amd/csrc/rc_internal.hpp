@@ -105,7 +105,16 @@ struct VariantInfo {
     bool fused;
     int lds_bytes;
     bool persistent;
+    bool wide = false;     // 16-pixel chunks, wide projection encoding
 };
+
+struct VariantEntry {
+    VariantInfo info;
+    hipError_t (*launch)(const CrossArgs &, hipStream_t);
+};
+
+// Wide kernels (both operands staged in LDS, 16-pixel chunks).
+const VariantEntry *wide_table(int form, int *n);
 
 int variant_count(int form, bool fused);
 const VariantInfo *variant_info(int form, bool fused, int idx);
@@ -121,6 +130,11 @@ hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const
 hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,
                           const float *eulers, int O, const float2 *coords, int npix, int kpad,
                           unsigned char *proj, float *pabs2, hipStream_t s);
+
+// Projections in the wide kernels' encoding: [C][kpad/16][O][16-pixel chunk].
+hipError_t launch_project_wide(int form, const ProjView *vols, const float *bscale, int n_classes,
+                               const float *eulers, int O, const float2 *coords, int npix,
+                               int kpad, unsigned char *proj, float *pabs2, hipStream_t s);
 
 // pnorm[i][c*O+o] = sum_p half_corr[i][p] * pabs2[c*O+o][p]
 hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,

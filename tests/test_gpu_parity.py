@@ -158,8 +158,14 @@ def test_run_diff2_kernel_coarse_rejects_3d_data(c1, lib, torch_cuda):
     assert e.value.status == 2
 
 
+def _wide_256(lib, form):
+    """Indices of the 256 x 256 kernels (the wide, LDS-staged family)."""
+    return [v for v, tm, tn in lib.variants(form) if tm == 256 and tn == 256]
+
+
+@pytest.mark.parametrize("kernel", ["default", "wide"])
 @pytest.mark.parametrize("form", ["fp32", "bf16x3"])
-def test_batched_cc(c1, lib, torch_cuda, form):
+def test_batched_cc(c1, lib, torch_cuda, form, kernel):
     """Batched cross-correlation scores; the per-particle minimum is the best
     correlation."""
     from relion_amd import synth
@@ -171,10 +177,15 @@ def test_batched_cc(c1, lib, torch_cuda, form):
     inv = np.empty_like(order)
     inv[order] = np.arange(order.size)
     projs = [lib.Projector(v) for v in prob.vols]
-    plan = lib.plan_for_problem(prob, projs, form, cc=True)
-    t = lib.problem_tensors(prob)
-    plan.execute(t)
-    torch.cuda.synchronize()
+    try:
+        if kernel == "wide":
+            lib.set_variant_override(_wide_256(lib, form)[0])
+        plan = lib.plan_for_problem(prob, projs, form, cc=True)
+        t = lib.problem_tensors(prob)
+        plan.execute(t)
+        torch.cuda.synchronize()
+    finally:
+        lib.set_variant_override(-1)
     d = t["diff2"].cpu().numpy().astype(np.float64)
     off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
     bidx = t["best_index"].cpu().numpy()
@@ -227,28 +238,36 @@ def test_batched_ragged_multiclass(lib, torch_cuda):
     prob = synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
                                np.concatenate(txs), np.concatenate(tys), pb, 32)
     off = lib.output_offsets(prob.trans_offset, 2, prob.orientation_num)
-    for form in ("fp32", "bf16x3"):
-        projs = [lib.Projector(v) for v in vols]
-        plan = lib.plan_for_problem(prob, projs, form)
-        t = lib.problem_tensors(prob)
-        plan.execute(t)
-        torch.cuda.synchronize()
-        d = t["diff2"].cpu().numpy().astype(np.float64)
-        bidx = t["best_index"].cpu().numpy()
-        for i in range(prob.n_particles):
-            ref = np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(2)])
-            _assert_close(d[off[i]:off[i + 1]], ref.ravel(), form, f"ragged {i}")
-            assert bidx[i] == oracle.argmin_flat(ref), (form, i)
+    refs = [np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(2)])
+            for i in range(prob.n_particles)]
+    projs = [lib.Projector(v) for v in vols]
+    try:
+        for form in ("fp32", "bf16x3"):
+            for v, tm, tn in lib.variants(form):
+                lib.set_variant_override(v)
+                plan = lib.plan_for_problem(prob, projs, form)
+                t = lib.problem_tensors(prob)
+                plan.execute(t)
+                torch.cuda.synchronize()
+                d = t["diff2"].cpu().numpy().astype(np.float64)
+                bidx = t["best_index"].cpu().numpy()
+                for i in range(prob.n_particles):
+                    _assert_close(d[off[i]:off[i + 1]], refs[i].ravel(), form,
+                                  f"ragged variant {v} ({tm}x{tn}) particle {i}")
+                    assert bidx[i] == oracle.argmin_flat(refs[i]), (form, v, i)
+                plan.close()
+    finally:
+        lib.set_variant_override(-1)
 
 
 def test_batched_variants_agree(c1, lib, torch_cuda):
-    """Every compiled tile shape gives the same FP32 result within tolerance."""
+    """Every compiled kernel and tile shape meets its form's tolerance."""
     torch = torch_cuda
     prob, ref = c1
     projs = [lib.Projector(v) for v in prob.vols]
     off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
     try:
-        for form in ("fp32", "bf16x3"):
+        for form in ("fp32", "bf16x3", "fp16"):
             for v, tm, tn in lib.variants(form):
                 lib.set_variant_override(v)
                 plan = lib.plan_for_problem(prob, projs, form)
@@ -263,13 +282,19 @@ def test_batched_variants_agree(c1, lib, torch_cuda):
         lib.set_variant_override(-1)
 
 
-def test_plan_graph_replay_matches_eager(c1, lib, torch_cuda):
+@pytest.mark.parametrize("kernel", ["default", "wide"])
+def test_plan_graph_replay_matches_eager(c1, lib, torch_cuda, kernel):
     """A captured plan replayed several times, with eager work and host
     synchronisations between replays, reproduces the eager result exactly."""
     torch = torch_cuda
     prob, ref = c1
     projs = [lib.Projector(v) for v in prob.vols]
-    plan = lib.plan_for_problem(prob, projs, "bf16x3")
+    try:
+        if kernel == "wide":
+            lib.set_variant_override(_wide_256(lib, "bf16x3")[-1])
+        plan = lib.plan_for_problem(prob, projs, "bf16x3")
+    finally:
+        lib.set_variant_override(-1)
     t = lib.problem_tensors(prob)
     plan.execute(t)
     torch.cuda.synchronize()
@@ -291,9 +316,21 @@ def test_plan_graph_replay_matches_eager(c1, lib, torch_cuda):
         torch.cuda.synchronize()
 
 
-def test_batched_deterministic_alone_and_in_batch(c1, lib, torch_cuda):
+@pytest.mark.parametrize("kernel", ["default", "wide"])
+def test_batched_deterministic_alone_and_in_batch(c1, lib, torch_cuda, kernel):
     """Each particle's diff2 is bit-identical whether it runs alone or inside
     a larger batch (no atomics, no split-K)."""
+    if kernel == "wide":
+        lib.set_variant_override(_wide_256(lib, "bf16x3")[1])
+        try:
+            _deterministic_alone_and_in_batch(c1, lib, torch_cuda)
+        finally:
+            lib.set_variant_override(-1)
+    else:
+        _deterministic_alone_and_in_batch(c1, lib, torch_cuda)
+
+
+def _deterministic_alone_and_in_batch(c1, lib, torch_cuda):
     from relion_amd import synth
     torch = torch_cuda
     prob, _ = c1
