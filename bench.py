@@ -51,8 +51,25 @@ def time_form(lib, torch, prob, projs, form, steps, warmup, tune=True):
     cross_ms = plan.timing_read(steps)
     res = dict(form=form, ms_per_step=(t1 - t0) * 1e3 / steps,
                cross_ms=sum(cross_ms) / len(cross_ms), variant=plan.tile(),
-               plan_bytes=plan.device_bytes(), autotuned=tuned is not None)
+               plan_bytes=plan.device_bytes(), autotuned=tuned is not None,
+               kernel=lib.variant_name(form, plan.tile()[0]))
     return res, t, plan
+
+
+def measured_traffic(kernel, form, prob, npix):
+    """HBM bytes per launch of the timed kernel on this workload, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (counters
+    cannot be read from inside the timed process); (None, None) if no
+    summary exists for this kernel, form and workload."""
+    from relion_amd import profiling
+    family = kernel.split("<", 1)[0]
+    work = f"{prob.n_particles}x{prob.orientation_num}x{int(prob.trans_of(0)[0].size)}x{npix}"
+    rel = os.path.join("profiles", f"hbm_traffic_{family}_{form}_{work}.md")
+    path = os.path.join(ROOT, rel)
+    if not os.path.exists(path):
+        return None, None
+    b = profiling.read_traffic(path)
+    return (b, rel) if b else (None, None)
 
 
 def cpu_baseline(prob, target_s):
@@ -190,6 +207,7 @@ def main():
     head = forms[0]
     r = results[head]
     peak = PEAK_TFLOPS[head]
+    traffic, traffic_src = measured_traffic(r["kernel"], head, prob, npix)
     line = {
         "metric": "diff2 evals/s @4608 orient×113 trans×1600 px; % MFMA peak; "
                   "streamed p99 ms",
@@ -210,8 +228,9 @@ def main():
                    "translations": int(prob.trans_of(0)[0].size), "npix": npix,
                    "classes": prob.n_classes, "tile": r["variant"]},
         "roofline": {"bound": "mfma", "achieved": r["cross_tflops"], "peak": peak,
-                     "unit": "TFLOP/s", "frac": r["cross_tflops"] / peak, "traffic": None,
-                     "kernel": f"cross_kernel<{head}>",
+                     "unit": "TFLOP/s", "frac": r["cross_tflops"] / peak, "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "kernel": r["kernel"],
                      "flops_per_launch": 4.0 * npix * evals},
         "forms": {f: {k: v for k, v in x.items()} for f, x in results.items()},
         "precision_vs_fp64": check,

@@ -208,6 +208,11 @@ int rc_set_variant_override(int variant);
 int rc_variant_count(int form);
 int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_gather);
 
+/* Human-readable description of a variant ("cross_kernel_wide<bf16x3>
+ * 256x256 ..."), naming the kernel template that a profiler lists; NULL for
+ * an unknown variant.  Diagnostic only, no counterpart in the reference. */
+const char *rc_variant_name(int form, int variant);
+
 /* ---------------------------------------------------------------------- */
 /* Replay of the reference's coarse-particle dumps                         */
 /* ---------------------------------------------------------------------- */

@@ -757,6 +757,23 @@ int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_
     return RC_OK;
 }
 
+const char *rc_variant_name(int form, int variant)
+{
+    const VariantInfo *v = variant_info(form, false, variant);
+    if (!v) return nullptr;
+    static const char *forms[] = {"fp32", "bf16x3", "fp16"};
+    static thread_local char buf[160];
+    snprintf(buf, sizeof(buf), "%s<%s> %dx%d, %d threads, %d B LDS%s%s", v->family,
+             forms[v->form], v->tile_m, v->tile_n, v->threads, v->lds_bytes,
+             v->persistent ? ", persistent" : "",
+             !v->wide             ? ""
+             : v->schedule == 1 ? ", staggered waves (bit 2)"
+             : v->schedule == 2 ? ", staggered waves (bit 0)"
+             : v->schedule == 4 ? ", fenced segments"
+                                : "");
+    return buf;
+}
+
 int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
 {
     RC_REQUIRE(out, RC_ERR_INVALID, "out is NULL");

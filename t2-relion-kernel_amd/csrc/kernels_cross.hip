@@ -779,8 +779,8 @@ template <int FORM, int WM, int WN, int FM, int FN, bool PERSIST>
 static VariantInfo info_pipe()
 {
     using Cfg = PipeCfg<FORM, WM, WN, FM, FN>;
-    return VariantInfo{FORM, Cfg::kTileM, Cfg::kTileN, Cfg::kThreads, WN, false, Cfg::kLds,
-                       PERSIST};
+    return VariantInfo{FORM,      Cfg::kTileM, Cfg::kTileN, Cfg::kThreads, WN, false,
+                       Cfg::kLds, PERSIST,     false,       "cross_kernel_pipe"};
 }
 
 #define RC_VARIANT(F, WM, WN, FM, FN, FU) \

@@ -595,7 +595,7 @@ static VariantEntry wide_one()
 {
     using Cfg = WideCfg<FORM, WM, WN, FM, FN>;
     return VariantEntry{VariantInfo{FORM, Cfg::kTileM, Cfg::kTileN, Cfg::kThreads, WN, false,
-                                    Cfg::kLds, false, true},
+                                    Cfg::kLds, false, true, "cross_kernel_wide", ST},
                         &launch_wide_one<FORM, WM, WN, FM, FN, ST>};
 }
 

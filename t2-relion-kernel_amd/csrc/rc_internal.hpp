@@ -106,6 +106,8 @@ struct VariantInfo {
     int lds_bytes;
     bool persistent;
     bool wide = false;     // 16-pixel chunks, wide projection encoding
+    const char *family = "cross_kernel";   // kernel template name
+    int schedule = 0;      // wide kernels: wave schedule (0 plain, 1/2 staggered, 4 fenced)
 };
 
 struct VariantEntry {

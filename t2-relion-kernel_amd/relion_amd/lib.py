@@ -35,7 +35,7 @@ EXPORTS = [
     "rc_plan_timing_read", "rc_autotune", "rc_plan_capture", "rc_plan_replay",
     "rc_server_create", "rc_server_submit", "rc_server_wait", "rc_server_diff2",
     "rc_server_release", "rc_server_stats", "rc_server_destroy", "rc_coarse_data_load",
-    "rc_coarse_data_free",
+    "rc_coarse_data_free", "rc_variant_name",
 ]
 
 
@@ -103,6 +103,8 @@ def load():
     L.rc_set_variant_override.argtypes = [i]
     L.rc_variant_count.argtypes = [i]
     L.rc_variant_info.argtypes = [i, i, C.POINTER(i), C.POINTER(i), C.POINTER(i)]
+    L.rc_variant_name.argtypes = [i, i]
+    L.rc_variant_name.restype = C.c_char_p
     L.rc_last_error.restype = C.c_char_p
     L.rc_version.restype = C.c_char_p
     _lib = L
@@ -213,6 +215,15 @@ def variants(form):
         check(L.rc_variant_info(f, v, C.byref(tm), C.byref(tn), C.byref(fu)))
         out.append((v, tm.value, tn.value))
     return out
+
+
+def variant_name(form, v):
+    """Kernel template and shape of variant ``v`` (as a profiler names it)."""
+    f = FORMS[form] if isinstance(form, str) else form
+    name = load().rc_variant_name(f, int(v))
+    if name is None:
+        raise RcError(f"no variant {v} for form {form}")
+    return name.decode()
 
 
 def set_variant_override(v):
