@@ -246,14 +246,21 @@ namespace rc {
 // for FP32 and BF16x3, pipelined 64 x 256 with one row fragment per wave
 // for FP16).
 // Measured on MI355X (512 particles x 113 translations x 4608 orientations x
-// 1636 pixels): the wide staggered 256 x 256 kernel is the fastest of every
-// form once its tiles fill the device; smaller searches use the pipelined
-// 128 x 128 kernel (FP16: 64 x 256), whose finer tiles waste less padding.
-static int default_variant(int form, int n_groups, int n_cols)
+// 1636 pixels): the wide 128 x 512 kernel with fenced segments is the
+// fastest of every form once its tiles fill the device and the 512-column
+// tiles waste little; then the wide staggered 256 x 256 one; smaller
+// searches use the pipelined 128 x 128 kernel (FP16: 64 x 256), whose finer
+// tiles waste less padding.  rc_autotune overrides this per shape.
+static int default_variant(int form, int n_groups, int n_classes, int O)
 {
-    constexpr int kWide = 14;   // first table entry past 10: wide, 256 x 256, staggered
-    const long long tiles = (long long)((n_groups * 16 + 255) / 256) * ((n_cols + 255) / 256);
-    if (tiles >= 512) return kWide;
+    constexpr int kWide256 = 14;   // wide, 256 x 256, staggered waves
+    constexpr int kWide512 = 21;   // wide, 128 x 512, fenced segments
+    const long long rows = (long long)n_groups * 16;
+    const long long t512 = ((rows + 127) / 128) * n_classes * ((O + 511) / 512);
+    const long long t256 = ((rows + 255) / 256) * n_classes * ((O + 255) / 256);
+    const bool narrow_waste = (long long)((O + 511) / 512) * 512 > O + O / 8;
+    if (t512 >= 1024 && !narrow_waste) return kWide512;
+    if (t256 >= 512) return kWide256;
     return form == kFP16 ? 7 : 4;
 }
 
@@ -335,7 +342,7 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(d));
         vi = it != g_tuned.end() ? it->second
-                                  : default_variant(p->form, p->n_groups, p->C * p->O);
+                                  : default_variant(p->form, p->n_groups, p->C, p->O);
     }
     const VariantInfo *info = variant_info(p->form, false, vi);
     RC_REQUIRE(info, RC_ERR_INVALID, "variant %d does not exist for form %d", vi, d->form);

@@ -158,9 +158,11 @@ __device__ __forceinline__ float2 a_value(const ARow &r, float x, float y, float
     return make_float2(c * w.x - s * w.y, c * w.y + s * w.x);
 }
 
-// Value j (0..7) of the thread's 8 pixels of a chunk.
-__device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[4],
-                                          const float4 (&cxy)[4], int j)
+// Value j of the thread's PPT consecutive pixels of a chunk (source data
+// as PPT/2 float4: two pixels each).
+template <int PPT>
+__device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[PPT / 2],
+                                          const float4 (&cxy)[PPT / 2], int j)
 {
     const float4 c = cxy[j >> 1], w = xw[j >> 1];
     const float x = (j & 1) ? c.z : c.x, y = (j & 1) ? c.w : c.y;
@@ -168,15 +170,18 @@ __device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[4],
     return a_value(ar, x, y, ww);
 }
 
-// Encodes the thread's 8 values (half h of row r) into LDS row buffer `dst`.
-template <int FORM>
-__device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[8], int h, int r,
+// Encodes the thread's PPT (8 or 4) values -- pixels PPT*h .. PPT*h+PPT-1 of
+// the chunk -- into LDS row buffer `dst` of row r.
+template <int FORM, int PPT>
+__device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], int h, int r,
                                         unsigned char *dst)
 {
+    static_assert(PPT == 8 || PPT == 4, "8 or 4 pixels per thread");
+    constexpr int G = PPT / 4;   // 4-pixel groups (one 16-byte segment each)
     if constexpr (FORM == kBF16X3) {
-        // segments 2h, 2h+1 (hi) and 4+2h, 5+2h (lo), 4 pixels (re, im) each
+        // group g: hi segment g, lo segment 4 + g; 4 pixels (re, im) each
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
+        for (int half = 0; half < G; ++half) {
             uint32_t hi[4], lo[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -189,15 +194,15 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[8], in
                 lo[i] = (uint32_t)__builtin_bit_cast(uint16_t, lr) |
                         ((uint32_t)__builtin_bit_cast(uint16_t, lm) << 16);
             }
-            const int sh = 2 * h + half, sl = 4 + 2 * h + half;
-            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, sh) * 16) =
+            const int g = G * h + half;
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, g) * 16) =
                 make_uint4(hi[0], hi[1], hi[2], hi[3]);
-            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, sl) * 16) =
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, 4 + g) * 16) =
                 make_uint4(lo[0], lo[1], lo[2], lo[3]);
         }
     } else if constexpr (FORM == kFP16) {
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
+        for (int half = 0; half < G; ++half) {
             uint32_t w[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -206,10 +211,10 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[8], in
                 w[i] = (uint32_t)__builtin_bit_cast(uint16_t, re) |
                        ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
             }
-            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, 2 * h + half) * 16) =
+            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, G * h + half) * 16) =
                 make_uint4(w[0], w[1], w[2], w[3]);
         }
-    } else {
+    } else if constexpr (PPT == 8) {
         // segment 2q + h holds component q % 2 of pixels 2i + q / 2, i = 0..3
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -222,19 +227,29 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[8], in
             *reinterpret_cast<float4 *>(dst + wide_swz<FORM>(r, 2 * q + h) * 16) =
                 make_float4(f[0], f[1], f[2], f[3]);
         }
+    } else {
+        // floats q*8 + 2h, q*8 + 2h + 1 hold component q % 2 of pixels q / 2
+        // and 2 + q / 2: 8 bytes in segment 2q + h/2 at byte (h & 1) * 8
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 z0 = v[q >> 1], z1 = v[2 + (q >> 1)];
+            const float2 f = (q & 1) ? make_float2(z0.y, z1.y) : make_float2(z0.x, z1.x);
+            *reinterpret_cast<float2 *>(dst + wide_swz<FORM>(r, 2 * q + (h >> 1)) * 16 +
+                                        (h & 1) * 8) = f;
+        }
     }
 }
 
-// Builds the thread's part (half h of the row) of a chunk into LDS row
-// buffer `dst` (row index r for the swizzle).
-template <int FORM>
-__device__ __forceinline__ void a_build(const ARow &ar, const float4 (&xw)[4],
-                                        const float4 (&cxy)[4], int h, int r, unsigned char *dst)
+// Builds the thread's part h of a chunk into LDS row buffer `dst` (row r).
+template <int FORM, int PPT>
+__device__ __forceinline__ void a_build(const ARow &ar, const float4 (&xw)[PPT / 2],
+                                        const float4 (&cxy)[PPT / 2], int h, int r,
+                                        unsigned char *dst)
 {
-    float2 v[8];
+    float2 v[PPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = a_pixel(ar, xw, cxy, j);
-    a_store<FORM>(ar, v, h, r, dst);
+    for (int j = 0; j < PPT; ++j) v[j] = a_pixel<PPT>(ar, xw, cxy, j);
+    a_store<FORM, PPT>(ar, v, h, r, dst);
 }
 
 // Operand registers of one 16-row (or 16-column) slice for one chunk.
@@ -307,6 +322,18 @@ __device__ __forceinline__ void wide_mfma(floatx4 (&acc)[FM], const WideFrag<FOR
     }
 }
 
+// Wait for every vector-memory operation but the newest 2 * (PPT / 2) (the
+// next chunk's source loads, issued last).
+template <int PPT>
+__device__ __forceinline__ void wait_vmcnt_keep()
+{
+    if constexpr (PPT == 8)
+        asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
+    else
+        asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
+}
+
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));   // dword-aligned 16 B
 typedef __attribute__((address_space(3))) void lds_void_w;
 typedef __attribute__((address_space(1))) const void gbl_void_w;
 
@@ -336,7 +363,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     constexpr int TN = Cfg::kTileN;
     constexpr int KB = Cfg::kKB;
     constexpr int SEGS = WideEnc<FORM>::kSegs;
-    static_assert(2 * TM == NT, "each thread builds half a row of a chunk");
+    constexpr int TPR = NT / TM;   // threads building one row of a chunk
+    constexpr int PPT = 16 / TPR;  // pixels each of them builds
+    static_assert(TPR * TM == NT && (TPR == 2 || TPR == 4), "2 or 4 threads per A row");
     constexpr int UNITS = TN * SEGS;
     static_assert(UNITS % NT == 0, "B tile must split evenly into 16-byte loads");
     constexpr int LPT = UNITS / NT;
@@ -355,13 +384,14 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     const int o0 = (ntile - cls * a.ntiles_per_class) * TN;
     const int O = a.orientation_num;
     const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int wm = wave % WM, wn = wave / WM;
     const int r16 = lane & 15, q = lane >> 4;
     const int nkc = a.kpad / 16;
 
     // ---- this thread's A row (row gr of the tile, pixels 8 gh .. 8 gh + 7 of a chunk)
-    const int gr = tid >> 1, gh = tid & 1;
+    const int gr = tid / TPR, gh = tid % TPR;
     ARow ar;
     {
         const int g = mtile * (TM / 16) + (gr >> 4);
@@ -386,9 +416,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
         ar.txr = ok ? (double)a.trans_x[base + t] * kInv2PiW : 0.0;
         ar.tyr = ok ? (double)a.trans_y[base + t] * kInv2PiW : 0.0;
         ar.scale = (FORM == kFP16) ? a.ascale[part] : 1.f;
-        ar.xw = a.xw + (size_t)part * a.kpad + 8 * gh;
+        ar.xw = a.xw + (size_t)part * a.kpad + PPT * gh;
     }
-    const float2 *c_row = a.coords + 8 * gh;
+    const float2 *c_row = a.coords + PPT * gh;
 
     floatx4 acc[FN][FM];
 #pragma unroll
@@ -412,11 +442,11 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             __builtin_amdgcn_global_load_lds((gbl_void_w *)src, (lds_void_w *)dst, 16, 0, 0);
         }
     };
-    auto load_src = [&](int kc, float4 (&xw)[4], float4 (&cxy)[4]) {
+    auto load_src = [&](int kc, float4 (&xw)[PPT / 2], float4 (&cxy)[PPT / 2]) {
         const float4 *xp = reinterpret_cast<const float4 *>(ar.xw + kc * 16);
         const float4 *cp = reinterpret_cast<const float4 *>(c_row + kc * 16);
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < PPT / 2; ++u) {
             xw[u] = xp[u];
             cxy[u] = cp[u];
         }
@@ -445,10 +475,10 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     const int nrun = (a.npix + 15) / 16;
     const bool lead = STAGGER == 0 || STAGGER == 4 ||
                       (((STAGGER == 1 ? (wave >> 2) : wave) & 1) == 0);
-    float4 xw[4], cxy[4];
+    float4 xw[PPT / 2], cxy[PPT / 2];
     issue_B(0, 0);
     load_src(0, xw, cxy);
-    a_build<FORM>(ar, xw, cxy, gh, gr, &sA[0][gr * KB]);
+    a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[0][gr * KB]);
     if (!lead || STAGGER == 4) load_src(min(1, nrun - 1), xw, cxy);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
@@ -473,7 +503,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             }
             WideFrag<FORM> fb_reg[2];
             wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
-            float2 v[8];
+            float2 v[PPT];
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int fb = 0; fb < FN; ++fb) {
@@ -483,30 +513,33 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                 }
                 wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg[fb & 1]);
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    if ((j * FN) / 8 == fb) v[j] = a_pixel(ar, xw, cxy, j);
+                for (int j = 0; j < PPT; ++j)
+                    if ((j * FN) / PPT == fb) v[j] = a_pixel<PPT>(ar, xw, cxy, j);
                 __builtin_amdgcn_sched_barrier(0);
             }
-            a_store<FORM>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
+            a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
             load_src(min(kn + 1, nrun - 1), xw, cxy);
         } else if (lead) {
             load_src(kn, xw, cxy);
             mfma_chunk(buf);
-            a_build<FORM>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+            a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
         } else {
-            a_build<FORM>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+            a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
             load_src(min(kn + 1, nrun - 1), xw, cxy);
             mfma_chunk(buf);
         }
         if (STAGGER == 4)
-            asm volatile("s_waitcnt vmcnt(8)" ::: "memory");   // the chunk's B copies
+            wait_vmcnt_keep<PPT>();   // the chunk's B copies; the next build's loads may fly
         else
             asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
-    // ---- epilogue: float64 assembly, stores, (min, argmin) partials
-    const int C = a.n_classes;
+    // The row groups' data for every fragment row of the wave, requested
+    // together before any is used.
+    int e_part[FM], e_t0[FM], e_T[FM], e_base[FM];
+    double e_xn[FM];
+    float e_inv[FM];
 #pragma unroll
     for (int fa = 0; fa < FM; ++fa) {
         const int g = mtile * (TM / 16) + wm * FM + fa;
@@ -526,38 +559,88 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                 T = a.n_trans;
             }
         }
+        e_part[fa] = part;
+        e_t0[fa] = t0;
+        e_T[fa] = T;
+        e_base[fa] = base;
+        e_xn[fa] = a.xnorm[part];
+        e_inv[fa] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
+    }
+
+    // ---- epilogue: float64 assembly, stores, (min, argmin) partials.
+    // Lane (r16, q) holds, for column slice fb, orientation o0 + rb0 + 16 fb +
+    // r16 and the four consecutive translations t0 + 4q .. t0 + 4q + 3 of row
+    // group fa: one 16-byte store when all four exist (global stores need
+    // only dword alignment).  A lane visits its elements in increasing flat
+    // index, so a strict "<" keeps the lowest index among equal minima.
+    const int C = a.n_classes;
+#pragma unroll
+    for (int fa = 0; fa < FM; ++fa) {
+        const int g = mtile * (TM / 16) + wm * FM + fa;
+        const int part = e_part[fa], t0 = e_t0[fa], T = e_T[fa], base = e_base[fa];
         const size_t out_base = (size_t)C * O * base + (size_t)cls * O * T;
-        const double xn = a.xnorm[part];
-        const float inv = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
+        const double xn = e_xn[fa];
+        const float inv = e_inv[fa];
+        // the row group's norms, all requested before any is used
+        float pnv[FN];
+#pragma unroll
+        for (int fb = 0; fb < FN; ++fb) {
+            const int o = min(o0 + rb0 + fb * 16 + r16, O - 1);
+            pnv[fb] = a.pnorm[((size_t)part * C + cls) * O + o];
+        }
+        const int tq = t0 + 4 * q;
+        const int nv = min(max(T - tq, 0), 4);
         double best = DBL_MAX;
-        long long best_k = 0x7fffffffffffffffLL;
+        int best_j = -1;
 #pragma unroll
         for (int fb = 0; fb < FN; ++fb) {
             const int o = o0 + rb0 + fb * 16 + r16;
-            if (o >= O) continue;
-            const float pn = a.pnorm[((size_t)part * C + cls) * O + o];
+            if (o >= O || nv == 0) continue;
+            const double pn = (double)pnv[fb];
+            const size_t k0 = out_base + (size_t)o * T + tq;
+            float init[4] = {0.f, 0.f, 0.f, 0.f};
+            if (!a.cc && a.diff2_init) {
+                if (nv == 4) {
+                    const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
+                    init[0] = x.x;
+                    init[1] = x.y;
+                    init[2] = x.z;
+                    init[3] = x.w;
+                } else {
+                    for (int v = 0; v < nv; ++v) init[v] = a.diff2_init[k0 + v];
+                }
+            }
+            const double sum = pn + xn;
+            const double rs = a.cc ? sqrt(2.0 * pn) : 1.0;
+            float out[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int t = t0 + 4 * q + v;
-                if (t >= T) continue;
-                const size_t k = out_base + (size_t)o * T + t;
                 const float cval = acc[fb][fa][v] * inv;
                 double d;
                 if (a.cc) {
-                    d = -(double)cval / sqrt(2.0 * (double)pn);
+                    d = -(double)cval / rs;
                 } else {
-                    d = (double)pn + xn - (double)cval;
-                    if (a.diff2_init) d += (double)a.diff2_init[k];
+                    d = sum - (double)cval;
+                    if (a.diff2_init) d += (double)init[v];
                 }
-                a.diff2[k] = (float)d;
-                const long long flat = (long long)(cls * O + o) * T + t;
-                if (d < best || (d == best && flat < best_k)) {
+                out[v] = (float)d;
+                if (v < nv && d < best) {
                     best = d;
-                    best_k = flat;
+                    best_j = 4 * fb + v;
                 }
+            }
+            if (nv == 4) {
+                *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
+            } else {
+                for (int v = 0; v < nv; ++v) a.diff2[k0 + v] = out[v];
             }
         }
         if (a.part_min) {
+            long long best_k = 0x7fffffffffffffffLL;
+            if (best_j >= 0) {
+                const int o = o0 + rb0 + (best_j >> 2) * 16 + r16;
+                best_k = (long long)(cls * O + o) * T + tq + (best_j & 3);
+            }
 #pragma unroll
             for (int off = 32; off > 0; off >>= 1) {
                 const double ob = __shfl_xor(best, off, 64);
@@ -610,6 +693,9 @@ static VariantEntry wide_one()
         wide_one<F, 4, 2, 4, 4, 1>(), /* 256 x 128, staggered by wave bit 2 */         \
         wide_one<F, 4, 2, 4, 8, 4>(), /* 256 x 256, fenced segments */                 \
         wide_one<F, 4, 2, 4, 4, 4>(), /* 256 x 128, fenced segments */                 \
+        wide_one<F, 2, 4, 4, 8, 0>(), /* 128 x 512, 512 threads */                     \
+        wide_one<F, 2, 4, 4, 8, 1>(), /* 128 x 512, staggered by wave bit 2 */         \
+        wide_one<F, 2, 4, 4, 8, 4>(), /* 128 x 512, fenced segments */                 \
     };
 
 RC_WIDE_TABLE(kFP32)
