@@ -295,6 +295,64 @@ int rc_server_release(rc_server *s, uint64_t ticket);
 int rc_server_stats(rc_server *s, rc_server_counters *out);
 int rc_server_destroy(rc_server *s);
 
+/* ------------------------------------------------------------------------ */
+/* Fine-pass diff2 over job lists.  Mirrors diff2_fine (utilities.h:213-235)
+ * and cuda_kernel_diff2_fine (diff2.cuh:540-682) for a 3D reference and 2D
+ * data: job b (b < todo_blocks) scores orientation d_rot_idx[d_job_idx[b]]
+ * against translations d_trans_idx[d_job_idx[b]] + j, j < d_job_num[b],
+ * writing d_diff2s[d_job_idx[b] + j] = sum_init + sum_p corr_p/2 |P - X_s|^2
+ * over image_size unfiltered pixels (written, not accumulated).  Pixel
+ * coordinates follow that kernel (rows between the half-planes parked at
+ * x = maxR).  A job whose orientation or translations fall outside
+ * orientation_num / translation_num is skipped.  d_diff2s must hold
+ * d_job_idx[b] + d_job_num[b] entries for every job, as in the reference.
+ * data_is_3D: RC_ERR_UNSUPPORTED. */
+int rc_run_diff2_kernel_fine(const rc_projector_kernel *projector, const float *d_eulers,
+                             const float *d_real, const float *d_imag, const float *d_trans_x,
+                             const float *d_trans_y, const float *d_trans_z, const float *d_corr,
+                             float *d_diff2s, unsigned long image_size, float sum_init,
+                             unsigned long orientation_num, unsigned long translation_num,
+                             unsigned long todo_blocks, const unsigned long *d_rot_idx,
+                             const unsigned long *d_trans_idx, const unsigned long *d_job_idx,
+                             const unsigned long *d_job_num, int data_is_3D, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* Coarse weights and significance: the step after the search.  Replaces the
+ * coarse weights exponent, exponentiate and significance-threshold steps of
+ * RELION's coarse pass (kernel_weights_exponent_coarse and
+ * kernel_exponentiate, utilities.h:405-459; array_over_threshold and
+ * find_threshold_idx_in_cumulative, cuda_device_utils.cuh:238-267).
+ *
+ * Per particle i and sample k = (c*O + o)*T_i + t of a batch's diff2:
+ *   lw_k = lp_orient[c][o] + lp_offset[c][t] + min_i - diff2_k   (float64),
+ *   w_k  = exp(lw_k - max lw)                                      (float),
+ * w_k = 0 where a log prior is -inf or diff2_k < (float)min_i;
+ *   sum_i = sum_k w_k (float64);  with the weights sorted ascending s[0..n-1]
+ *   and c[j] = s[0] + ... + s[j],  m = #{j : c[j] <= (1 - fraction) * sum_i},
+ *   significant_weight_i = s[min(m, n-1)], and sample k is significant when
+ *   w_k >= significant_weight_i.
+ * Deterministic; one workgroup per particle; no workspace. */
+typedef struct rc_weights_desc {
+    int n_particles, n_classes, orientation_num;
+    const int *host_trans_offset;     /* host [n_particles+1] (shape checks)     */
+    const int *d_trans_offset;        /* device copy                             */
+    const float *d_diff2;             /* device, rc_batch_desc.d_diff2 layout     */
+    const double *d_min_diff2;        /* device [n_particles] (the plan's best)  */
+    const float *d_log_prior_orient;  /* device [n_classes*O] per particle, or NULL */
+    long long log_prior_orient_stride;/* elements between particles (0: shared)  */
+    const float *d_log_prior_offset;  /* device, per particle [n_classes][T_i] at
+                                         n_classes*trans_offset[i], or NULL       */
+    double adaptive_fraction;         /* in (0, 1], RELION's adaptive_fraction    */
+    float *d_weights;                 /* device out, diff2 layout (may alias d_diff2) */
+    double *d_sum_weight;             /* device out [n_particles]                */
+    float *d_significant_weight;      /* device out [n_particles]                */
+    int *d_n_significant;             /* device out [n_particles]                */
+    unsigned char *d_significant;     /* device out, diff2 layout, 1 = significant, or NULL */
+    double *d_max_log_weight;         /* device out [n_particles] (max lw), or NULL */
+} rc_weights_desc;
+
+int rc_coarse_weights(const rc_weights_desc *desc, void *stream);
+
 /* ---------------------------------------------------------------------- */
 const char *rc_last_error(void);
 const char *rc_version(void);

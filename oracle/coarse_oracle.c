@@ -309,3 +309,57 @@ void oracle_cc_f64(const float *vre, const float *vim, int mdlX, int mdlY, int m
         }
     }
 }
+
+/* Fine-pass diff2 over job lists (cuda_kernel_diff2_fine, diff2.cuh:540-682):
+ * job b scores orientation rot_idx[job_idx[b]] against translations
+ * trans_idx[job_idx[b]] + j, j < job_num[b]:
+ *   out[job_idx[b] + j] = sum_init + sum_p corr_p/2 |P(p) - X_p e^{i phi}|^2
+ * over the unfiltered image, with that kernel's coordinate rule (rows between
+ * the half-planes parked at x = maxR, diff2.cuh:593-617).  float64. */
+void oracle_fine_f64(const float *vre, const float *vim, int mdlX, int mdlY, int mdlZ, int initY,
+                     int initZ, float pad, int maxR2_padded, int imgX, int imgY, int maxR,
+                     const float *eulers, const float *trans_x, const float *trans_y,
+                     const float *re, const float *im, const float *corr, int image_size,
+                     double sum_init, int n_jobs, const long long *rot_idx,
+                     const long long *trans_idx, const long long *job_idx,
+                     const long long *job_num, double *out)
+{
+    vol_t v = {vre, vim, mdlX, mdlY, mdlZ, initY, initZ};
+#pragma omp parallel
+    {
+        double *pr = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *pi = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *xs = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        double *ys = (double *)__builtin_alloca(sizeof(double) * (size_t)image_size);
+        for (int p = 0; p < image_size; ++p) {
+            int x = p % imgX, y = p / imgX;
+            if (y > maxR) {
+                if (y >= imgY - maxR)
+                    y -= imgY;
+                else
+                    x = maxR;
+            }
+            xs[p] = x;
+            ys[p] = y;
+        }
+#pragma omp for schedule(dynamic, 1)
+        for (int b = 0; b < n_jobs; ++b) {
+            const long long j0 = job_idx[b];
+            const float *e = eulers + 9 * (size_t)rot_idx[j0];
+            for (int p = 0; p < image_size; ++p)
+                project_f64(&v, (double)pad, maxR2_padded, e, xs[p], ys[p], pr + p, pi + p);
+            for (long long j = 0; j < job_num[b]; ++j) {
+                const long long it = trans_idx[j0] + j;
+                double acc = 0.0;
+                for (int p = 0; p < image_size; ++p) {
+                    double phi = xs[p] * (double)trans_x[it] + ys[p] * (double)trans_y[it];
+                    double s = sin(phi), c = cos(phi);
+                    double dr = pr[p] - (c * re[p] - s * im[p]);
+                    double di = pi[p] - (c * im[p] + s * re[p]);
+                    acc += (dr * dr + di * di) * 0.5 * (double)corr[p];
+                }
+                out[j0 + j] = acc + sum_init;
+            }
+        }
+    }
+}

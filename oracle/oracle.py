@@ -54,6 +54,12 @@ def lib():
                                     C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _f32p, C.c_int,
                                     _f32p, _f32p, C.c_int, _f32p, _f32p, _f32p, C.c_int, _f64p]
         L.oracle_cc_f64.restype = None
+        _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
+        L.oracle_fine_f64.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _f32p,
+                                      _f32p, _f32p, _f32p, _f32p, _f32p, C.c_int, C.c_double,
+                                      C.c_int, _i64p, _i64p, _i64p, _i64p, _f64p]
+        L.oracle_fine_f64.restype = None
         _LIB = L
     return _LIB
 
@@ -132,6 +138,23 @@ def cc_f64(v, maxR2_padded, imgX, imgY, maxR, eulers, trans_x, trans_y, re, im, 
     return out.reshape(O, T)
 
 
+def fine_f64(v, maxR2_padded, imgX, imgY, maxR, eulers, trans_x, trans_y, re, im, corr,
+             rot_idx, trans_idx, job_idx, job_num, sum_init=0.0, n_out=None):
+    """float64 fine-pass diff2 of one unfiltered image over a job list;
+    returns ``out`` with out[job_idx[b] + j] filled (NaN elsewhere)."""
+    i64 = lambda a: np.ascontiguousarray(a, np.int64)
+    rot_idx, trans_idx, job_idx, job_num = map(i64, (rot_idx, trans_idx, job_idx, job_num))
+    n = int(np.size(re))
+    if n_out is None:
+        n_out = int((job_idx + job_num).max()) if job_num.size else 0
+    out = np.full(n_out, np.nan)
+    lib().oracle_fine_f64(*_vol_args(v), int(maxR2_padded), imgX, imgY, maxR,
+                          _c32(eulers).ravel(), _c32(trans_x), _c32(trans_y), _c32(re), _c32(im),
+                          _c32(corr), n, float(sum_init), int(job_num.size), rot_idx, trans_idx,
+                          job_idx, job_num, out)
+    return out
+
+
 def problem_diff2(prob, i, cls=0, orient=None, init=None):
     """float64 diff2 ``[O, T_i]`` of particle ``i`` against class ``cls`` of a
     synthetic problem (filtered pixel order, tail = out-of-radius)."""
@@ -160,3 +183,53 @@ def argmin_flat(d):
     """Lowest flat index of the minimum (ties broken by the lower index)."""
     d = np.asarray(d).ravel()
     return int(np.argmin(d))
+
+
+# ---------------------------------------------------------------------------
+# coarse weights and significance (the step after the search)
+# ---------------------------------------------------------------------------
+def significance_threshold(w, fraction, total=None):
+    """RELION's significance rule on one particle's weights: sort ascending,
+    inclusive cumulative sums c, threshold (1 - fraction) * total, index
+    i + 1 of the pair c[i] <= threshold < c[i + 1] (0 if none), and the
+    weight at that index (find_threshold_idx_in_cumulative,
+    cuda_device_utils.cuh:251-262; array_over_threshold :238-249).  Written
+    as the literal scan over pairs so it restates the reference's rule.
+    Returns (significant weight, number of weights >= it)."""
+    s = np.sort(np.asarray(w, np.float64).ravel())
+    c = np.cumsum(s)
+    total = c[-1] if total is None else float(total)
+    thr = (1.0 - fraction) * total
+    idx = 0
+    hit = np.nonzero((c[:-1] <= thr) & (thr < c[1:]))[0]
+    if hit.size:
+        idx = int(hit[0]) + 1
+    ws = s[idx]
+    return ws, int(np.count_nonzero(np.asarray(w).ravel() >= ws))
+
+
+def coarse_weights(diff2, min_diff2, n_classes, orientation_num, T, fraction,
+                   log_prior_orient=None, log_prior_offset=None):
+    """float64 weights of one particle from its coarse diff2 ``[C*O*T]``
+    (utilities.h:405-459 semantics): lw = lp_orient + lp_offset + min - diff2,
+    w = exp(lw - max lw); samples with a -inf log prior or diff2 below
+    (float)min get w = 0.  Returns (w, sum, significant weight, count,
+    max lw)."""
+    C, O = n_classes, orientation_num
+    d = np.asarray(diff2, np.float64).reshape(C, O, T)
+    lpo = (np.zeros((C, O, 1)) if log_prior_orient is None
+           else np.asarray(log_prior_orient, np.float64).reshape(C, O, 1))
+    lpt = (np.zeros((C, 1, T)) if log_prior_offset is None
+           else np.asarray(log_prior_offset, np.float64).reshape(C, 1, T))
+    ok = (d >= np.float32(min_diff2)) & np.isfinite(lpo) & np.isfinite(lpt)
+    with np.errstate(invalid="ignore"):
+        lw = lpo + lpt + (float(min_diff2) - d)
+    if not ok.any():
+        w = np.zeros_like(d)
+        ws, n = significance_threshold(w, fraction, 0.0)
+        return w.ravel(), 0.0, ws, n, -np.inf
+    lmax = lw[ok].max()
+    w = np.where(ok, np.exp(np.where(ok, lw - lmax, 0.0)), 0.0)
+    total = float(w.sum())
+    ws, n = significance_threshold(w, fraction, total)
+    return w.ravel(), total, ws, n, lmax

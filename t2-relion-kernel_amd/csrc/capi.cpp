@@ -764,6 +764,93 @@ int rc_variant_info(int form, int variant, int *tile_m, int *tile_n, int *fused_
     return RC_OK;
 }
 
+int rc_run_diff2_kernel_fine(const rc_projector_kernel *projector, const float *d_eulers,
+                             const float *d_real, const float *d_imag, const float *d_trans_x,
+                             const float *d_trans_y, const float *d_trans_z, const float *d_corr,
+                             float *d_diff2s, unsigned long image_size, float sum_init,
+                             unsigned long orientation_num, unsigned long translation_num,
+                             unsigned long todo_blocks, const unsigned long *d_rot_idx,
+                             const unsigned long *d_trans_idx, const unsigned long *d_job_idx,
+                             const unsigned long *d_job_num, int data_is_3D, void *stream)
+{
+    (void)d_trans_z;
+    RC_REQUIRE(!data_is_3D, RC_ERR_UNSUPPORTED, "3D data (data_is_3D) is not implemented");
+    if (int st = check_kernel_view(projector)) return st;
+    RC_REQUIRE(image_size > 0 && image_size <= (unsigned long)kMaxPix, RC_ERR_LIMIT,
+               "image_size %lu out of range", image_size);
+    RC_REQUIRE(image_size <= (unsigned long)projector->imgX * projector->imgY, RC_ERR_INVALID,
+               "image_size %lu exceeds imgX*imgY", image_size);
+    RC_REQUIRE(orientation_num > 0 && translation_num > 0, RC_ERR_INVALID,
+               "orientation_num and translation_num must be positive");
+    RC_REQUIRE(todo_blocks < (1ul << 31), RC_ERR_LIMIT, "too many jobs");
+    if (todo_blocks == 0) return RC_OK;
+    RC_REQUIRE(d_eulers && d_real && d_imag && d_trans_x && d_trans_y && d_corr && d_diff2s &&
+                   d_rot_idx && d_trans_idx && d_job_idx && d_job_num,
+               RC_ERR_INVALID, "NULL device pointer");
+    FineArgs a{};
+    a.v = view_of(*projector);
+    a.imgX = projector->imgX;
+    a.imgY = projector->imgY;
+    a.maxR = projector->maxR;
+    a.eulers = d_eulers;
+    a.real = d_real;
+    a.imag = d_imag;
+    a.corr = d_corr;
+    a.trans_x = d_trans_x;
+    a.trans_y = d_trans_y;
+    a.diff2 = d_diff2s;
+    a.image_size = (int)image_size;
+    a.sum_init = sum_init;
+    a.orientation_num = orientation_num;
+    a.translation_num = translation_num;
+    a.rot_idx = d_rot_idx;
+    a.trans_idx = d_trans_idx;
+    a.job_idx = d_job_idx;
+    a.job_num = d_job_num;
+    RC_HIP(launch_fine(a, todo_blocks, (hipStream_t)stream));
+    return RC_OK;
+}
+
+int rc_coarse_weights(const rc_weights_desc *d, void *stream)
+{
+    RC_REQUIRE(d, RC_ERR_INVALID, "NULL weights descriptor");
+    RC_REQUIRE(d->n_particles > 0 && d->n_classes > 0 && d->orientation_num > 0,
+               RC_ERR_INVALID, "particles, classes and orientations must be positive");
+    RC_REQUIRE(d->adaptive_fraction > 0.0 && d->adaptive_fraction <= 1.0, RC_ERR_INVALID,
+               "adaptive_fraction %g outside (0, 1]", d->adaptive_fraction);
+    RC_REQUIRE(d->host_trans_offset && d->d_trans_offset && d->d_diff2 && d->d_min_diff2 &&
+                   d->d_weights && d->d_sum_weight && d->d_significant_weight &&
+                   d->d_n_significant,
+               RC_ERR_INVALID, "NULL pointer in weights descriptor");
+    RC_REQUIRE(d->log_prior_orient_stride >= 0, RC_ERR_INVALID, "negative prior stride");
+    for (int i = 0; i < d->n_particles; ++i) {
+        const long long T = (long long)d->host_trans_offset[i + 1] - d->host_trans_offset[i];
+        RC_REQUIRE(T > 0 && T <= kMaxTrans, RC_ERR_INVALID,
+                   "particle %d has %lld translations", i, T);
+        RC_REQUIRE((long long)d->n_classes * d->orientation_num * T < (1ll << 31),
+                   RC_ERR_LIMIT, "particle %d has more than 2^31 samples", i);
+    }
+    WeightsArgs a{};
+    a.n_particles = d->n_particles;
+    a.n_classes = d->n_classes;
+    a.orientation_num = d->orientation_num;
+    a.trans_offset = d->d_trans_offset;
+    a.diff2 = d->d_diff2;
+    a.min_diff2 = d->d_min_diff2;
+    a.lp_orient = d->d_log_prior_orient;
+    a.orient_stride = d->log_prior_orient_stride;
+    a.lp_offset = d->d_log_prior_offset;
+    a.fraction = d->adaptive_fraction;
+    a.weights = d->d_weights;
+    a.sum_weight = d->d_sum_weight;
+    a.significant_weight = d->d_significant_weight;
+    a.n_significant = d->d_n_significant;
+    a.mask = d->d_significant;
+    a.max_log_weight = d->d_max_log_weight;
+    RC_HIP(launch_weights(a, (hipStream_t)stream));
+    return RC_OK;
+}
+
 const char *rc_variant_name(int form, int variant)
 {
     const VariantInfo *v = variant_info(form, false, variant);

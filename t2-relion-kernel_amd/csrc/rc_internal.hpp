@@ -147,6 +147,40 @@ hipError_t launch_argmin(int n_particles, const int *group_begin, int parts_per_
                          const double *part_min, const long long *part_idx, double *best,
                          long long *best_idx, hipStream_t s);
 
+// Coarse weights and significance of a batch (one workgroup per particle).
+struct WeightsArgs {
+    int n_particles, n_classes, orientation_num;
+    const int *trans_offset;     // [P+1] device
+    const float *diff2;          // batch layout of the search output
+    const double *min_diff2;     // [P]
+    const float *lp_orient;      // [C][O] per particle at i * orient_stride, or NULL
+    long long orient_stride;
+    const float *lp_offset;      // [C][T_i] at C * trans_offset[i], or NULL
+    double fraction;             // adaptive fraction of the weight kept significant
+    float *weights;              // batch layout (may alias diff2)
+    double *sum_weight;          // [P]
+    float *significant_weight;   // [P]
+    int *n_significant;          // [P]
+    unsigned char *mask;         // batch layout or NULL
+    double *max_log_weight;      // [P] or NULL
+};
+hipError_t launch_weights(const WeightsArgs &a, hipStream_t s);
+
+// Fine-pass diff2 over job lists (one workgroup per job).
+struct FineArgs {
+    ProjView v;
+    int imgX, imgY, maxR;
+    const float *eulers;
+    const float *real, *imag, *corr;
+    const float *trans_x, *trans_y;
+    float *diff2;
+    int image_size;
+    float sum_init;
+    unsigned long orientation_num, translation_num;
+    const unsigned long *rot_idx, *trans_idx, *job_idx, *job_num;
+};
+hipError_t launch_fine(const FineArgs &a, unsigned long n_jobs, hipStream_t s);
+
 // Direct-form float32 vector kernel (the original kernel's arithmetic).
 hipError_t launch_direct(const ProjView &v, const float *eulers, int O, const float *trans_x,
                          const float *trans_y, int T, const float2 *coords, const float *real,

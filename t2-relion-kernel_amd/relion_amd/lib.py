@@ -35,7 +35,8 @@ EXPORTS = [
     "rc_plan_timing_read", "rc_autotune", "rc_plan_capture", "rc_plan_replay",
     "rc_server_create", "rc_server_submit", "rc_server_wait", "rc_server_diff2",
     "rc_server_release", "rc_server_stats", "rc_server_destroy", "rc_coarse_data_load",
-    "rc_coarse_data_free", "rc_variant_name",
+    "rc_coarse_data_free", "rc_variant_name", "rc_coarse_weights",
+    "rc_run_diff2_kernel_fine",
 ]
 
 
@@ -61,6 +62,17 @@ class rc_batch_desc(C.Structure):
                 ("d_diff2_init", C.c_void_p), ("d_diff2", C.c_void_p),
                 ("d_best_diff2", C.c_void_p), ("d_best_index", C.c_void_p),
                 ("do_cc", C.c_int)]
+
+
+class rc_weights_desc(C.Structure):
+    _fields_ = [("n_particles", C.c_int), ("n_classes", C.c_int), ("orientation_num", C.c_int),
+                ("host_trans_offset", C.POINTER(C.c_int)), ("d_trans_offset", C.c_void_p),
+                ("d_diff2", C.c_void_p), ("d_min_diff2", C.c_void_p),
+                ("d_log_prior_orient", C.c_void_p), ("log_prior_orient_stride", C.c_longlong),
+                ("d_log_prior_offset", C.c_void_p), ("adaptive_fraction", C.c_double),
+                ("d_weights", C.c_void_p), ("d_sum_weight", C.c_void_p),
+                ("d_significant_weight", C.c_void_p), ("d_n_significant", C.c_void_p),
+                ("d_significant", C.c_void_p), ("d_max_log_weight", C.c_void_p)]
 
 
 _lib = None
@@ -90,6 +102,10 @@ def load():
     L.rc_coarse_run.argtypes = [i, i, i, i, i, vp, vp, vp, vp, vp,
                                 C.POINTER(rc_projector_kernel), vp, vp, vp, vp, vp, vp, sz]
     L.rc_plan_create.argtypes = [C.POINTER(rc_batch_desc), C.POINTER(vp)]
+    L.rc_coarse_weights.argtypes = [C.POINTER(rc_weights_desc), vp]
+    ul = C.c_ulong
+    L.rc_run_diff2_kernel_fine.argtypes = [C.POINTER(rc_projector_kernel)] + [vp] * 8 + [
+        ul, f, ul, ul, ul] + [vp] * 4 + [i, vp]
     L.rc_plan_execute.argtypes = [vp, C.POINTER(rc_batch_desc), vp]
     L.rc_plan_destroy.argtypes = [vp]
     L.rc_plan_device_bytes.argtypes = [vp]
@@ -332,6 +348,63 @@ def problem_tensors(prob, device="cuda"):
         diff2=torch.empty(total, dtype=torch.float32, device=device),
         best_diff2=torch.empty(prob.n_particles, dtype=torch.float64, device=device),
         best_index=torch.empty(prob.n_particles, dtype=torch.int64, device=device))
+
+
+def run_diff2_kernel_fine(pk, eulers, real, imag, trans_x, trans_y, corr, diff2, image_size,
+                          sum_init, orientation_num, translation_num, rot_idx, trans_idx,
+                          job_idx, job_num, data_is_3D=False, stream=None):
+    """rc_run_diff2_kernel_fine on torch device tensors (index arrays int64,
+    passed as unsigned long)."""
+    import torch
+    L = load()
+    p = lambda x: None if x is None else x.data_ptr()
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(L.rc_run_diff2_kernel_fine(C.byref(pk), p(eulers), p(real), p(imag), p(trans_x),
+                                     p(trans_y), None, p(corr), p(diff2), int(image_size),
+                                     float(sum_init), int(orientation_num), int(translation_num),
+                                     int(job_num.numel()), p(rot_idx), p(trans_idx), p(job_idx),
+                                     p(job_num), 1 if data_is_3D else 0, s))
+
+
+def coarse_weights(trans_offset, d_trans_offset, n_classes, orientation_num, diff2, min_diff2,
+                   log_prior_orient=None, orient_stride=0, log_prior_offset=None,
+                   fraction=0.999, weights=None, mask=True, stream=None):
+    """rc_coarse_weights on torch device tensors; returns a dict with
+    ``weights`` (float32, diff2 layout), ``sum`` (float64 [P]),
+    ``significant_weight`` (float32 [P]), ``n_significant`` (int32 [P]),
+    ``mask`` (uint8, diff2 layout, or None) and ``max_log_weight``."""
+    import torch
+    L = load()
+    P = len(trans_offset) - 1
+    dev = diff2.device
+    if weights is None:
+        weights = torch.empty_like(diff2)
+    out = dict(weights=weights,
+               sum=torch.empty(P, dtype=torch.float64, device=dev),
+               significant_weight=torch.empty(P, dtype=torch.float32, device=dev),
+               n_significant=torch.empty(P, dtype=torch.int32, device=dev),
+               mask=torch.empty(diff2.numel(), dtype=torch.uint8, device=dev) if mask else None,
+               max_log_weight=torch.empty(P, dtype=torch.float64, device=dev))
+    off = np.ascontiguousarray(trans_offset, np.int32)
+    d = rc_weights_desc()
+    d.n_particles, d.n_classes, d.orientation_num = P, n_classes, orientation_num
+    d.host_trans_offset = off.ctypes.data_as(C.POINTER(C.c_int))
+    d.d_trans_offset = d_trans_offset.data_ptr()
+    d.d_diff2 = diff2.data_ptr()
+    d.d_min_diff2 = min_diff2.data_ptr()
+    d.d_log_prior_orient = log_prior_orient.data_ptr() if log_prior_orient is not None else None
+    d.log_prior_orient_stride = int(orient_stride)
+    d.d_log_prior_offset = log_prior_offset.data_ptr() if log_prior_offset is not None else None
+    d.adaptive_fraction = float(fraction)
+    d.d_weights = weights.data_ptr()
+    d.d_sum_weight = out["sum"].data_ptr()
+    d.d_significant_weight = out["significant_weight"].data_ptr()
+    d.d_n_significant = out["n_significant"].data_ptr()
+    d.d_significant = out["mask"].data_ptr() if mask else None
+    d.d_max_log_weight = out["max_log_weight"].data_ptr()
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(L.rc_coarse_weights(C.byref(d), s))
+    return out
 
 
 def plan_for_problem(prob, projectors, form, cc=False):

@@ -1,0 +1,128 @@
+"""rc_coarse_weights (GPU) against the float64 oracle: weights, their sum,
+the significance threshold and mask (utilities.h:405-459,
+cuda_device_utils.cuh:238-267 semantics)."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _search(lib, torch, prob, form="fp32"):
+    projs = [lib.Projector(v) for v in prob.vols]
+    plan = lib.plan_for_problem(prob, projs, form)
+    t = lib.problem_tensors(prob)
+    plan.execute(t)
+    torch.cuda.synchronize()
+    return t
+
+
+def _priors(prob, rng, excluded=0.05):
+    C, O = prob.n_classes, prob.orientation_num
+    lpo = rng.normal(scale=2.0, size=C * O).astype(np.float32)
+    lpo[rng.random(lpo.size) < excluded] = -np.inf
+    lpt = rng.normal(scale=1.0, size=C * int(prob.trans_offset[-1])).astype(np.float32)
+    lpt[rng.random(lpt.size) < excluded] = -np.inf
+    return lpo, lpt
+
+
+def _check(prob, t, out, fraction, lpo, lpt):
+    C, O = prob.n_classes, prob.orientation_num
+    off = np.asarray(prob.trans_offset, np.int64)
+    d = t["diff2"].cpu().numpy()
+    best = t["best_diff2"].cpu().numpy()
+    w = out["weights"].cpu().numpy()
+    mask = out["mask"].cpu().numpy()
+    S = out["sum"].cpu().numpy()
+    ws = out["significant_weight"].cpu().numpy()
+    ns = out["n_significant"].cpu().numpy()
+    for i in range(prob.n_particles):
+        T = int(off[i + 1] - off[i])
+        a, b = C * O * off[i], C * O * off[i + 1]
+        pt = None if lpt is None else lpt[C * off[i]:C * off[i + 1]]
+        rw, rS, rws, rn, _ = oracle.coarse_weights(d[a:b], best[i], C, O, T, fraction, lpo, pt)
+        gw = w[a:b].astype(np.float64)
+        big = rw > 1e-30
+        assert np.all(np.abs(gw[big] - rw[big]) <= 2e-7 * rw[big]), i
+        assert np.all(np.abs(gw[~big]) <= 1e-30), i
+        assert S[i] == pytest.approx(rS, rel=2e-7)   # float-stored weights
+        assert S[i] == pytest.approx(gw.sum(), rel=1e-12)
+        # the selection rule applied to the GPU's own weights gives its answer
+        gws, gn = oracle.significance_threshold(gw, fraction, S[i])
+        assert (np.float32(gws), gn) == (ws[i], ns[i]), (i, gws, gn, ws[i], ns[i])
+        assert np.array_equal(mask[a:b], (w[a:b] >= ws[i]).astype(np.uint8))
+        # and the oracle's float64 weights select (almost) the same samples
+        assert abs(int(ns[i]) - rn) <= max(2, rn // 1000), (i, ns[i], rn)
+
+
+@pytest.fixture(scope="module")
+def lib(torch_cuda):
+    from relion_amd import lib as L
+    L.load()
+    return L
+
+
+@pytest.mark.parametrize("fraction", [0.999, 0.9, 1.0])
+def test_weights_c1(lib, torch_cuda, fraction):
+    from relion_amd import synth
+    torch = torch_cuda
+    prob = synth.make_problem("c1", seed=11)
+    t = _search(lib, torch, prob)
+    rng = np.random.default_rng(5)
+    lpo, lpt = _priors(prob, rng)
+    dlpo = torch.as_tensor(lpo).cuda()
+    dlpt = torch.as_tensor(lpt).cuda()
+    out = lib.coarse_weights(prob.trans_offset, t["trans_offset"], 1, prob.orientation_num,
+                             t["diff2"], t["best_diff2"], dlpo, 0, dlpt, fraction)
+    torch.cuda.synchronize()
+    _check(prob, t, out, fraction, lpo, lpt)
+
+
+def test_weights_ragged_multiclass_no_priors_in_place(lib, torch_cuda):
+    """Two classes, uneven translation counts, no priors, weights written
+    over diff2; a second run and a one-particle batch agree bit for bit."""
+    from relion_amd import synth
+    torch = torch_cuda
+    vols = [synth.fourier_volume(synth.blob_map(32, seed=s)) for s in (3, 4)]
+    g = synth.CoarseGeometry(imgY=16, maxR=8)
+    eul = synth.orientation_set(0, 8)
+    pb = synth.make_particles(vols, g, 5, 32, 0.5, seed=9, shift_range=1.0)
+    txs, tys, offs = [], [], [0]
+    for i, r2 in enumerate((1, 13, 4, 2, 0)):
+        tx, ty, _ = synth.translation_set(r2, 32, centre=(0.5 * i, -0.25 * i))
+        txs.append(tx)
+        tys.append(ty)
+        offs.append(offs[-1] + tx.size)
+    prob = synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
+                               np.concatenate(txs), np.concatenate(tys), pb, 32)
+    t = _search(lib, torch, prob)
+    keep = t["diff2"].clone()
+    out = lib.coarse_weights(prob.trans_offset, t["trans_offset"], 2, prob.orientation_num,
+                             keep.clone(), t["best_diff2"], fraction=0.99)
+    inplace = keep.clone()
+    out2 = lib.coarse_weights(prob.trans_offset, t["trans_offset"], 2, prob.orientation_num,
+                              inplace, t["best_diff2"], fraction=0.99, weights=inplace)
+    torch.cuda.synchronize()
+    _check(prob, t, out, 0.99, None, None)
+    for k in ("weights", "sum", "significant_weight", "n_significant", "mask"):
+        assert torch.equal(out[k], out2[k]), k
+    # particle 3 alone
+    i = 3
+    a, b = 2 * prob.orientation_num * offs[i], 2 * prob.orientation_num * offs[i + 1]
+    one_off = np.array([0, offs[i + 1] - offs[i]], np.int32)
+    one = lib.coarse_weights(one_off, torch.as_tensor(one_off).cuda(), 2, prob.orientation_num,
+                             keep[a:b].clone(), t["best_diff2"][i:i + 1].clone(), fraction=0.99)
+    torch.cuda.synchronize()
+    assert torch.equal(one["weights"], out["weights"][a:b])
+    assert torch.equal(one["significant_weight"], out["significant_weight"][i:i + 1])
+    assert torch.equal(one["n_significant"], out["n_significant"][i:i + 1])
+
+
+def test_weights_rejects_bad_fraction(lib, torch_cuda):
+    torch = torch_cuda
+    off = np.array([0, 2], np.int32)
+    x = torch.zeros(4, device="cuda")
+    m = torch.zeros(1, dtype=torch.float64, device="cuda")
+    with pytest.raises(lib.RcError):
+        lib.coarse_weights(off, torch.as_tensor(off).cuda(), 1, 2, x, m, fraction=0.0)
