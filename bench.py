@@ -144,6 +144,109 @@ def serve_bench(n_requests, rate_hz, forms, slots, budget_gb):
     return out
 
 
+def downstream_bench(lib, torch, prob, t, steps, cpu_seconds):
+    """The two steps after the search, each timed with HIP events on the
+    launch stream: (1) coarse weights + significance (fraction 0.999) over
+    the whole batch's diff2; (2) the fine pass of one particle: 1 % of its
+    coarse (orientation, translation) pairs significant, each expanded to 8
+    finer orientations (HEALPix order 3 x 48 psi) x 4 finer translations
+    (quarter-pixel offsets), one job per (fine orientation, coarse pair)."""
+    import numpy as np
+    from relion_amd import synth
+    out = {}
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    P = prob.n_particles
+    samples = t["diff2"].numel()
+    # ---- weights and significance
+    w = lib.coarse_weights(prob.trans_offset, t["trans_offset"], prob.n_classes,
+                           prob.orientation_num, t["diff2"], t["best_diff2"], fraction=0.999)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(steps):
+        lib.coarse_weights(prob.trans_offset, t["trans_offset"], prob.n_classes,
+                           prob.orientation_num, t["diff2"], t["best_diff2"], fraction=0.999,
+                           weights=w["weights"], mask=True)
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / steps
+    algo = 9.0 * samples           # read diff2, write weight and mask byte
+    nsig = w["n_significant"].cpu().numpy()
+    out["weights"] = dict(
+        kernel="weights_kernel", ms=ms, samples_per_s=samples / (ms * 1e-3),
+        roofline=dict(bound="hbm", achieved=algo / (ms * 1e-3) / 1e9, peak=8000.0, unit="GB/s",
+                      frac=algo / (ms * 1e-3) / 1e9 / 8000.0,
+                      bytes_per_launch=algo, note="algorithmic bytes: 4 B read + 5 B written "
+                      "per sample; the kernel reads each weight once more per selection pass"),
+        significant_per_particle_median=float(np.median(nsig)),
+        workload=f"{P} particles x {samples // P} samples, adaptive fraction 0.999")
+    # ---- fine pass of particle 0
+    g = prob.geom
+    order, _ = synth.filter_order(g)
+    inv = np.empty_like(order)
+    inv[order] = np.arange(order.size)
+    pb = prob.particles
+    fine_eul = synth.orientation_set(3, 48)                  # 8 per coarse orientation
+    tx, ty = prob.trans_of(0)
+    q = 2 * np.pi * 0.25 / prob.box
+    ftx = np.stack([tx - q, tx + q, tx - q, tx + q], 1).ravel().astype(np.float32)
+    fty = np.stack([ty - q, ty - q, ty + q, ty + q], 1).ravel().astype(np.float32)
+    rng = np.random.default_rng(1)
+    O, T = prob.orientation_num, tx.size
+    pairs = np.flatnonzero(rng.random(O * T) < 0.01)
+    rot, trans, job_idx, job_num = [], [], [], []
+    for k in pairs:
+        o, tt = divmod(int(k), T)
+        for sub in range(8):
+            job_idx.append(len(rot))
+            job_num.append(4)
+            for u in range(4):
+                rot.append(8 * o + sub)
+                trans.append(4 * tt + u)
+    dev = lambda a, dt: torch.as_tensor(np.asarray(a)).to("cuda", dt)
+    rot_d, trans_d = dev(rot, torch.int64), dev(trans, torch.int64)
+    jidx_d, jnum_d = dev(job_idx, torch.int64), dev(job_num, torch.int64)
+    re = dev(pb.real[0][inv], torch.float32)
+    im = dev(pb.imag[0][inv], torch.float32)
+    corr = dev(pb.corr[0][inv], torch.float32)
+    eul_d = dev(fine_eul.ravel(), torch.float32)
+    ftx_d, fty_d = dev(ftx, torch.float32), dev(fty, torch.float32)
+    fout = torch.empty(len(rot), dtype=torch.float32, device="cuda")
+    proj = lib.Projector(prob.vols[0])
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    run = lambda: lib.run_diff2_kernel_fine(pk, eul_d, re, im, ftx_d, fty_d, corr, fout,
+                                            g.image_size, 0.0, fine_eul.shape[0], ftx.size,
+                                            rot_d, trans_d, jidx_d, jnum_d)
+    run()
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    for _ in range(steps):
+        run()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    fms = ev[0].elapsed_time(ev[1]) / steps
+    fevals = len(rot)
+    fine = dict(kernel="fine_kernel", ms=fms, evals_per_s=fevals / (fms * 1e-3),
+                pixel_evals_per_s=fevals * g.image_size / (fms * 1e-3),
+                workload=f"1 particle, {len(job_idx)} jobs x 4 translations = {fevals} fine "
+                         f"evaluations over {g.image_size} pixels")
+    if cpu_seconds > 0:
+        from oracle import oracle
+        nj = len(job_idx)
+        take = max(1, min(nj, int(nj * min(1.0, cpu_seconds / 60.0))))
+        t0 = time.perf_counter()
+        oracle.fine_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, fine_eul, ftx, fty,
+                        pb.real[0][inv], pb.imag[0][inv], pb.corr[0][inv], rot, trans,
+                        job_idx[:take], job_num[:take], n_out=len(rot))
+        dt = time.perf_counter() - t0
+        fine["cpu_port_evals_per_s"] = 4 * take / dt
+        fine["cpu_port_sample"] = (f"{take} jobs, float64 oracle with OpenMP on "
+                                   f"{os.cpu_count()} host threads, {dt:.1f} s")
+    out["fine"] = fine
+    proj.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,11 +263,13 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
+    ap.add_argument("--no-downstream", action="store_true",
+                    help="skip timing the weights/significance and fine-pass steps")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding of "
-                                   "particle batches is described in DESIGN.md but not built",
-                          "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; particle batches would "
+                                   "shard across GPUs with no exchange on the hot path, "
+                                   "which is not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
@@ -203,6 +308,11 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(prob, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+    downstream = None
+    if not args.no_downstream:
+        downstream = downstream_bench(lib, torch, prob, outs[forms[0]], args.steps,
+                                      0.0 if args.no_cpu_baseline else 5.0)
+        log(f"downstream: {downstream}")
 
     head = forms[0]
     r = results[head]
@@ -235,6 +345,7 @@ def main():
         "forms": {f: {k: v for k, v in x.items()} for f, x in results.items()},
         "precision_vs_fp64": check,
         "streamed": streamed,
+        "downstream": downstream,
         "cpu_baseline": cpu,
     }
     if "fp32" in results and head != "fp32":
