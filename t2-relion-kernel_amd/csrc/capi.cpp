@@ -251,11 +251,11 @@ namespace rc {
 // tiles waste little; then the wide staggered 256 x 256 one; smaller
 // searches use the pipelined 128 x 128 kernel (FP16: 64 x 256), whose finer
 // tiles waste less padding.  rc_autotune overrides this per shape.
-static int default_variant(int form, int n_groups, int n_classes, int O)
+static int default_variant(int form, int n_rows, int n_classes, int O)
 {
     constexpr int kWide256 = 14;   // wide, 256 x 256, staggered waves
     constexpr int kWide512 = 21;   // wide, 128 x 512, fenced segments
-    const long long rows = (long long)n_groups * 16;
+    const long long rows = n_rows;   // the wide kernels pack the rows
     const long long t512 = ((rows + 127) / 128) * n_classes * ((O + 511) / 512);
     const long long t256 = ((rows + 255) / 256) * n_classes * ((O + 255) / 256);
     const bool narrow_waste = (long long)((O + 511) / 512) * 512 > O + O / 8;
@@ -297,6 +297,7 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     const size_t nt_total = (size_t)p->C * p->ntiles_per_class;
     p->groups = c.take<int2>(p->n_groups);
     p->group_begin = c.take<int>(p->P + 1);
+    p->row_part = c.take<int>(p->n_rows);
     p->vols = c.take<ProjView>(p->C);
     p->bscale = c.take<float>(p->C);
     p->xw = c.take<float2>((size_t)p->P * p->kpad);
@@ -307,8 +308,9 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     p->proj = c.take<unsigned char>((size_t)p->C * p->O * p->nkc * chunk_bytes(p->form));
     p->pabs2 = c.take<float>((size_t)p->C * p->O * p->kpad);
     p->pnorm = c.take<float>((size_t)p->P * p->C * p->O);
-    p->part_min = c.take<double>((size_t)p->n_groups * nt_total * p->parts_per_group);
-    p->part_idx = c.take<long long>((size_t)p->n_groups * nt_total * p->parts_per_group);
+    const size_t prows = p->packed ? (size_t)p->n_rows : (size_t)p->n_groups;
+    p->part_min = c.take<double>(prows * nt_total * p->parts_per_group);
+    p->part_idx = c.take<long long>(prows * nt_total * p->parts_per_group);
     return align_up(c.off, 256);
 }
 
@@ -337,18 +339,24 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
     }
     tab.group_begin[p->P] = (int)tab.groups.size();
     p->n_groups = (int)tab.groups.size();
+    p->n_rows = p->trans_offset[p->P];
+    tab.row_part.resize(p->n_rows);
+    for (int i = 0; i < p->P; ++i)
+        for (int r = p->trans_offset[i]; r < p->trans_offset[i + 1]; ++r) tab.row_part[r] = i;
     int vi = g_variant_override;
     if (vi < 0) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(d));
         vi = it != g_tuned.end() ? it->second
-                                  : default_variant(p->form, p->n_groups, p->C, p->O);
+                                  : default_variant(p->form, p->trans_offset[p->P], p->C, p->O);
     }
     const VariantInfo *info = variant_info(p->form, false, vi);
     RC_REQUIRE(info, RC_ERR_INVALID, "variant %d does not exist for form %d", vi, d->form);
     p->variant = vi;
     p->info = *info;
-    p->n_mtiles = (p->n_groups * 16 + info->tile_m - 1) / info->tile_m;
+    p->packed = info->wide;
+    const long long rows = p->packed ? (long long)p->n_rows : (long long)p->n_groups * 16;
+    p->n_mtiles = (int)((rows + info->tile_m - 1) / info->tile_m);
     p->ntiles_per_class = (p->O + info->tile_n - 1) / info->tile_n;
     p->parts_per_group = info->waves_n;
     {
@@ -377,6 +385,7 @@ size_t plan_table_bytes(const PlanTables &tab)
 {
     return align_up(tab.groups.size() * sizeof(int2), 256) +
            align_up(tab.group_begin.size() * sizeof(int), 256) +
+           align_up(tab.row_part.size() * sizeof(int), 256) +
            align_up(tab.views.size() * sizeof(ProjView), 256) +
            align_up(tab.scales.size() * sizeof(float), 256);
 }
@@ -392,6 +401,7 @@ int plan_upload_async(rc_plan *p, const PlanTables &tab, unsigned char *staging,
     };
     RC_HIP(put(p->groups, tab.groups.data(), tab.groups.size() * sizeof(int2)));
     RC_HIP(put(p->group_begin, tab.group_begin.data(), tab.group_begin.size() * sizeof(int)));
+    RC_HIP(put(p->row_part, tab.row_part.data(), tab.row_part.size() * sizeof(int)));
     RC_HIP(put(p->vols, tab.views.data(), tab.views.size() * sizeof(ProjView)));
     RC_HIP(put(p->bscale, tab.scales.data(), tab.scales.size() * sizeof(float)));
     return RC_OK;
@@ -428,6 +438,8 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
     CrossArgs a{};
     a.groups = p->groups;
     a.n_groups = p->n_groups;
+    a.row_part = p->row_part;
+    a.n_rows = p->n_rows;
     a.trans_offset = d->d_trans_offset;
     a.n_trans = 0;
     a.trans_x = d->d_trans_x;
@@ -462,7 +474,7 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
         ++p->launches;
     }
     if (want_best)
-        RC_HIP(launch_argmin(p->P, p->group_begin,
+        RC_HIP(launch_argmin(p->P, p->packed ? d->d_trans_offset : p->group_begin,
                              p->C * p->ntiles_per_class * p->parts_per_group, p->part_min,
                              p->part_idx, d->d_best_diff2, (long long *)d->d_best_index, s));
     return RC_OK;
@@ -889,6 +901,8 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
     if (hipMemcpy(p->groups, tab.groups.data(), tab.groups.size() * sizeof(int2),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->group_begin, tab.group_begin.data(), tab.group_begin.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->row_part, tab.row_part.data(), tab.row_part.size() * sizeof(int),
                   hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(p->vols, tab.views.data(), tab.views.size() * sizeof(ProjView),
                   hipMemcpyHostToDevice) != hipSuccess ||

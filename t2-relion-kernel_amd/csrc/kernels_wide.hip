@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <type_traits>
 
 #include "device_common.hpp"
 
@@ -337,6 +338,15 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));   // dword-al
 typedef __attribute__((address_space(3))) void lds_void_w;
 typedef __attribute__((address_space(1))) const void gbl_void_w;
 
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
 template <int FORM, int WM, int WN, int FM, int FN>
 struct WideCfg {
     static constexpr int kThreads = 64 * WM * WN;
@@ -392,29 +402,17 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
 
     // ---- this thread's A row (row gr of the tile, pixels 8 gh .. 8 gh + 7 of a chunk)
     const int gr = tid / TPR, gh = tid % TPR;
+    // Rows are packed: row R of the batch is translation R of the CSR
+    // translation arrays (particle row_part[R]), so no particle is padded to a
+    // multiple of 16 rows.  Rows past the last one are built from particle 0
+    // with zero shift and never stored.
     ARow ar;
     {
-        const int g = mtile * (TM / 16) + (gr >> 4);
-        int part = 0, t0 = 0, T = 0, base = 0;
-        if (g < a.n_groups) {
-            if (a.groups) {
-                const int2 gi = a.groups[g];
-                part = gi.x;
-                t0 = gi.y;
-            } else {
-                t0 = 16 * g;
-            }
-            if (a.trans_offset) {
-                base = a.trans_offset[part];
-                T = a.trans_offset[part + 1] - base;
-            } else {
-                T = a.n_trans;
-            }
-        }
-        const int t = t0 + (gr & 15);
-        const bool ok = t < T;
-        ar.txr = ok ? (double)a.trans_x[base + t] * kInv2PiW : 0.0;
-        ar.tyr = ok ? (double)a.trans_y[base + t] * kInv2PiW : 0.0;
+        const int R = mtile * TM + gr;
+        const bool ok = R < a.n_rows;
+        const int part = ok ? a.row_part[R] : 0;
+        ar.txr = ok ? (double)a.trans_x[R] * kInv2PiW : 0.0;
+        ar.tyr = ok ? (double)a.trans_y[R] * kInv2PiW : 0.0;
         ar.scale = (FORM == kFP16) ? a.ascale[part] : 1.f;
         ar.xw = a.xw + (size_t)part * a.kpad + PPT * gh;
     }
@@ -535,128 +533,127 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
         __syncthreads();
     }
 
-    // The row groups' data for every fragment row of the wave, requested
-    // together before any is used.
-    int e_part[FM], e_t0[FM], e_T[FM], e_base[FM];
-    double e_xn[FM];
-    float e_inv[FM];
-#pragma unroll
-    for (int fa = 0; fa < FM; ++fa) {
-        const int g = mtile * (TM / 16) + wm * FM + fa;
-        int part = 0, t0 = 0, T = 0, base = 0;
-        if (g < a.n_groups) {
-            if (a.groups) {
-                const int2 gi = a.groups[g];
-                part = gi.x;
-                t0 = gi.y;
-            } else {
-                t0 = 16 * g;
-            }
-            if (a.trans_offset) {
-                base = a.trans_offset[part];
-                T = a.trans_offset[part + 1] - base;
-            } else {
-                T = a.n_trans;
-            }
-        }
-        e_part[fa] = part;
-        e_t0[fa] = t0;
-        e_T[fa] = T;
-        e_base[fa] = base;
-        e_xn[fa] = a.xnorm[part];
-        e_inv[fa] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
-    }
-
-    // ---- epilogue: float64 assembly, stores, (min, argmin) partials.
+    // ---- epilogue: float64 assembly, stores, per-row (min, argmin) partials.
     // Lane (r16, q) holds, for column slice fb, orientation o0 + rb0 + 16 fb +
-    // r16 and the four consecutive translations t0 + 4q .. t0 + 4q + 3 of row
-    // group fa: one 16-byte store when all four exist (global stores need
-    // only dword alignment).  A lane visits its elements in increasing flat
-    // index, so a strict "<" keeps the lowest index among equal minima.
+    // r16 and the four consecutive rows R = R0 + 4q + v of fragment row fa.
+    // When the four rows belong to one particle they are four consecutive
+    // translations, i.e. 16 contiguous output bytes: one store (global
+    // stores need only dword alignment).  A lane visits its columns in
+    // increasing orientation, so a strict "<" keeps the lowest flat index
+    // among equal minima of a row.
     const int C = a.n_classes;
+    // one call per fragment row, with fa a compile-time constant so that the
+    // accumulators stay in registers
+    auto epilogue_row = [&](auto fa_c) {
+        constexpr int fa = decltype(fa_c)::value;
+        const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
+        int rpart[4], rT[4], rt[4];
+        size_t rbase[4];
+        double rxn[4];
+        float rinv[4];
+        bool rok[4];
 #pragma unroll
-    for (int fa = 0; fa < FM; ++fa) {
-        const int g = mtile * (TM / 16) + wm * FM + fa;
-        const int part = e_part[fa], t0 = e_t0[fa], T = e_T[fa], base = e_base[fa];
-        const size_t out_base = (size_t)C * O * base + (size_t)cls * O * T;
-        const double xn = e_xn[fa];
-        const float inv = e_inv[fa];
-        // the row group's norms, all requested before any is used
-        float pnv[FN];
-#pragma unroll
-        for (int fb = 0; fb < FN; ++fb) {
-            const int o = min(o0 + rb0 + fb * 16 + r16, O - 1);
-            pnv[fb] = a.pnorm[((size_t)part * C + cls) * O + o];
+        for (int v = 0; v < 4; ++v) {
+            const int R = R0 + v;
+            rok[v] = R < a.n_rows;
+            const int part = rok[v] ? a.row_part[R] : 0;
+            const int b0 = a.trans_offset[part];
+            rpart[v] = part;
+            rT[v] = a.trans_offset[part + 1] - b0;
+            rt[v] = R - b0;
+            rbase[v] = (size_t)C * O * b0 + (size_t)cls * O * rT[v];
+            rxn[v] = a.xnorm[part];
+            rinv[v] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
         }
-        const int tq = t0 + 4 * q;
-        const int nv = min(max(T - tq, 0), 4);
-        double best = DBL_MAX;
-        int best_j = -1;
+        const bool same = rok[3] && rpart[0] == rpart[3];   // rows are sorted by particle
+        double best[4];
+        int best_fb[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            best[v] = DBL_MAX;
+            best_fb[v] = -1;
+        }
 #pragma unroll
         for (int fb = 0; fb < FN; ++fb) {
             const int o = o0 + rb0 + fb * 16 + r16;
-            if (o >= O || nv == 0) continue;
-            const double pn = (double)pnv[fb];
-            const size_t k0 = out_base + (size_t)o * T + tq;
+            if (o >= O || !rok[0]) continue;
+            float out[4];
+            double pn[4];
+            if (same) {
+                const double p = (double)a.pnorm[((size_t)rpart[0] * C + cls) * O + o];
+                pn[0] = pn[1] = pn[2] = pn[3] = p;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    pn[v] = rok[v] ? (double)a.pnorm[((size_t)rpart[v] * C + cls) * O + o] : 0.0;
+            }
             float init[4] = {0.f, 0.f, 0.f, 0.f};
+            const size_t k0 = rbase[0] + (size_t)o * rT[0] + rt[0];
             if (!a.cc && a.diff2_init) {
-                if (nv == 4) {
+                if (same) {
                     const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
                     init[0] = x.x;
                     init[1] = x.y;
                     init[2] = x.z;
                     init[3] = x.w;
                 } else {
-                    for (int v = 0; v < nv; ++v) init[v] = a.diff2_init[k0 + v];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        if (rok[v]) init[v] = a.diff2_init[rbase[v] + (size_t)o * rT[v] + rt[v]];
                 }
             }
-            const double sum = pn + xn;
-            const double rs = a.cc ? sqrt(2.0 * pn) : 1.0;
-            float out[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float cval = acc[fb][fa][v] * inv;
+                const float cval = acc[fb][fa][v] * rinv[v];
                 double d;
                 if (a.cc) {
-                    d = -(double)cval / rs;
+                    d = -(double)cval / sqrt(2.0 * pn[v]);
                 } else {
-                    d = sum - (double)cval;
+                    d = pn[v] + rxn[v] - (double)cval;
                     if (a.diff2_init) d += (double)init[v];
                 }
                 out[v] = (float)d;
-                if (v < nv && d < best) {
-                    best = d;
-                    best_j = 4 * fb + v;
+                if (rok[v] && d < best[v]) {
+                    best[v] = d;
+                    best_fb[v] = fb;
                 }
             }
-            if (nv == 4) {
+            if (same) {
                 *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
             } else {
-                for (int v = 0; v < nv; ++v) a.diff2[k0 + v] = out[v];
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    if (rok[v]) a.diff2[rbase[v] + (size_t)o * rT[v] + rt[v]] = out[v];
             }
         }
         if (a.part_min) {
-            long long best_k = 0x7fffffffffffffffLL;
-            if (best_j >= 0) {
-                const int o = o0 + rb0 + (best_j >> 2) * 16 + r16;
-                best_k = (long long)(cls * O + o) * T + tq + (best_j & 3);
-            }
+            // per row: reduce over the 16 lanes that share q (the columns)
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                const double ob = __shfl_xor(best, off, 64);
-                const long long ok = __shfl_xor(best_k, off, 64);
-                if (ob < best || (ob == best && ok < best_k)) {
-                    best = ob;
-                    best_k = ok;
+            for (int v = 0; v < 4; ++v) {
+                double bv = best[v];
+                long long bk = 0x7fffffffffffffffLL;
+                if (best_fb[v] >= 0) {
+                    const int o = o0 + rb0 + best_fb[v] * 16 + r16;
+                    bk = (long long)(cls * O + o) * rT[v] + rt[v];
+                }
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) {
+                    const double ob = __shfl_xor(bv, off, 64);
+                    const long long ok2 = __shfl_xor(bk, off, 64);
+                    if (ob < bv || (ob == bv && ok2 < bk)) {
+                        bv = ob;
+                        bk = ok2;
+                    }
+                }
+                if (r16 == 0 && rok[v]) {
+                    const size_t u = ((size_t)(R0 + v) * n_ntiles + ntile) * WN + wn;
+                    a.part_min[u] = bv;
+                    a.part_idx[u] = bk;
                 }
             }
-            if (lane == 0 && g < a.n_groups) {
-                const size_t u = ((size_t)g * n_ntiles + ntile) * WN + wn;
-                a.part_min[u] = best;
-                a.part_idx[u] = best_k;
-            }
         }
-    }
+    };
+    static_for<0, FM>(epilogue_row);
 }
 
 // ---------------------------------------------------------------------------

@@ -15,10 +15,13 @@ struct rc_plan {
     std::vector<int> trans_offset;          // host copy
     std::vector<int> proj_dims;             // per class: 7 ints for shape checks
     int n_groups, n_mtiles, ntiles_per_class, parts_per_group;
+    int n_rows;          // sum of the translation counts
+    bool packed;         // wide kernels: rows packed without per-particle padding
     unsigned char *mem = nullptr;
     size_t bytes = 0;
     int2 *groups;
     int *group_begin;
+    int *row_part;
     rc::ProjView *vols;
     float *bscale;
     float2 *xw, *coords;
@@ -42,6 +45,7 @@ namespace rc {
 struct PlanTables {
     std::vector<int2> groups;
     std::vector<int> group_begin;
+    std::vector<int> row_part;
     std::vector<ProjView> views;
     std::vector<float> scales;
 };

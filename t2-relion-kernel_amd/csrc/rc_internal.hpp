@@ -67,6 +67,9 @@ struct CrossArgs {
     // rows: row group g covers translations t0 .. t0+15 of one particle
     const int2 *groups;          // [n_groups] (particle, t0); NULL => particle 0, t0 = 16 g
     int n_groups;
+    // packed rows (wide kernels): row R is translation R of the CSR arrays
+    const int *row_part;         // [n_rows] particle of each row
+    int n_rows;
     const int *trans_offset;     // [P+1]; NULL => single particle with T = n_trans
     int n_trans;
     const float *trans_x, *trans_y;
