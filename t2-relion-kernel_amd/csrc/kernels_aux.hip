@@ -172,32 +172,65 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
 // ---------------------------------------------------------------------------
 // A small GEMM on the matrix cores (v_mfma_f32_16x16x4_f32): block tile
 // 64 particles x 64 columns, 4 waves of 32 x 32, k staged through LDS in
-// 32-pixel chunks.  A[i][k] = half_corr, B[k][n] = |P|^2.
+// 64-pixel chunks (rows padded to 68 floats: 16-byte stores, conflict-free
+// column reads); the next chunk's global loads are issued before the
+// current chunk's MFMAs.  A[i][k] = half_corr, B[k][n] = |P|^2.  kpad is a
+// multiple of 32; a last half chunk is zero-filled.
+constexpr int kPnK = 64;
+constexpr int kPnRow = kPnK + 4;
+
 __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
                                                          const float *__restrict__ hc,
                                                          const float *__restrict__ pabs2,
                                                          float *__restrict__ pnorm)
 {
-    __shared__ float sA[64][33];
-    __shared__ float sB[64][33];
-    const int i0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __shared__ __attribute__((aligned(16))) float sA[64][kPnRow];
+    __shared__ __attribute__((aligned(16))) float sB[64][kPnRow];
+    // XCD-aware order: block b runs on XCD b % 8; each XCD takes a contiguous
+    // range of tiles in which the particle tiles of one column tile are
+    // adjacent, so that column tile's |P|^2 rows come from its L2
+    const int nrow = (P + 63) / 64, ncol = (N + 63) / 64;
+    const int total = nrow * ncol;
+    const int per_xcd = (total + 7) / 8;
+    const int L = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+    if (L >= total) return;
+    const int i0 = (L % nrow) * 64, n0 = (L / nrow) * 64;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int r16 = lane & 15, q = lane >> 4;
+    // this thread's share of a chunk: row lr, k offsets lk .. lk + 15
+    const int lr = tid >> 2, lk = (tid & 3) * 16;
+    const bool a_ok = i0 + lr < P, b_ok = n0 + lr < N;
+    const float *ap = hc + (size_t)(a_ok ? i0 + lr : 0) * kpad + lk;
+    const float *bp = pabs2 + (size_t)(b_ok ? n0 + lr : 0) * kpad + lk;
+    const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 ra[4], rb[4];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool in = k0 + lk + 4 * u < kpad;
+            ra[u] = (a_ok && in) ? *reinterpret_cast<const float4 *>(ap + k0 + 4 * u) : zero;
+            rb[u] = (b_ok && in) ? *reinterpret_cast<const float4 *>(bp + k0 + 4 * u) : zero;
+        }
+    };
     floatx4 acc[2][2];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < kpad; k0 += 32) {
-        for (int u = threadIdx.x; u < 64 * 32; u += 256) {
-            const int r = u >> 5, k = u & 31;
-            sA[r][k] = (i0 + r < P) ? hc[(size_t)(i0 + r) * kpad + k0 + k] : 0.f;
-            sB[r][k] = (n0 + r < N) ? pabs2[(size_t)(n0 + r) * kpad + k0 + k] : 0.f;
-        }
+        for (int y = 0; y < 2; ++y) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
+    fetch(0);
+    for (int k0 = 0; k0 < kpad; k0 += kPnK) {
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
+        for (int u = 0; u < 4; ++u) {
+            *reinterpret_cast<float4 *>(&sA[lr][lk + 4 * u]) = ra[u];
+            *reinterpret_cast<float4 *>(&sB[lr][lk + 4 * u]) = rb[u];
+        }
+        __syncthreads();
+        if (k0 + kPnK < kpad) fetch(k0 + kPnK);
+#pragma unroll
+        for (int s = 0; s < kPnK / 4; ++s) {
             const int k = 4 * s + q;
             const float a0 = sA[wr + r16][k], a1 = sA[wr + 16 + r16][k];
             const float b0 = sB[wc + r16][k], b1 = sB[wc + 16 + r16][k];
@@ -206,24 +239,24 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
             acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
-        __syncthreads();
     }
     // D[row = 4q + v][col = r16] of each 16 x 16 fragment
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int i = i0 + wr + 16 * a + 4 * q + v, n = n0 + wc + 16 * b + r16;
-                if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[a][b][v];
+                const int i = i0 + wr + 16 * x + 4 * q + v, n = n0 + wc + 16 * y + r16;
+                if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[x][y][v];
             }
 }
 
 hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,
                         const float *pabs2, float *pnorm, hipStream_t s)
 {
-    dim3 grid((n_cols + 63) / 64, (n_particles + 63) / 64);
+    const int total = ((n_cols + 63) / 64) * ((n_particles + 63) / 64);
+    dim3 grid(((total + 7) / 8) * 8);
     hipLaunchKernelGGL(pnorm_mfma_kernel, grid, dim3(256), 0, s, n_particles, n_cols, kpad,
                        half_corr, pabs2, pnorm);
     return hipGetLastError();
