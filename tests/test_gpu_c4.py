@@ -1,0 +1,40 @@
+"""The 8-class ragged workload (translations 49-197 per particle around
+random prior centres, 4608 orientations, 64^2 coarse images) on a reduced
+particle count: every form's arg-min agrees, and three particles (smallest,
+largest and a middle translation count) match the float64 oracle over all
+classes."""
+import numpy as np
+import pytest
+
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c4_reduced_against_oracle(torch_cuda):
+    from relion_amd import lib, synth
+    torch = torch_cuda
+    prob = synth.make_problem("c4", particles=40, seed=21)
+    C, O = prob.n_classes, prob.orientation_num
+    T = np.diff(prob.trans_offset)
+    assert C == 8 and T.min() >= 49 and T.max() <= 197
+    projs = [lib.Projector(v) for v in prob.vols]
+    off = lib.output_offsets(prob.trans_offset, C, O)
+    picks = [int(np.argmin(T)), int(np.argmax(T)), int(np.argsort(T)[len(T) // 2])]
+    refs = {i: np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(C)]) for i in picks}
+    best = {}
+    for form, (mx, mean) in (("bf16x3", (1e-5, 2e-6)), ("fp32", (2e-5, 1e-6))):
+        plan = lib.plan_for_problem(prob, projs, form)
+        print(form, "variant", plan.tile())
+        t = lib.problem_tensors(prob)
+        plan.execute(t)
+        torch.cuda.synchronize()
+        d = t["diff2"].cpu().numpy()
+        best[form] = t["best_index"].cpu().numpy()
+        for i in picks:
+            st = oracle.error_stats(d[off[i]:off[i + 1]], refs[i].ravel())
+            print(form, i, int(T[i]), st["max_rel"], st["mean_rel"])
+            assert st["max_rel"] <= mx and st["mean_rel"] <= mean, (form, i, st)
+            assert best[form][i] == oracle.argmin_flat(refs[i]), (form, i)
+        plan.close()
+    assert np.array_equal(best["bf16x3"], best["fp32"])
