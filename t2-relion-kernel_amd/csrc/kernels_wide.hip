@@ -6,13 +6,15 @@
 // 16x16x32 BF16/FP16 MFMA (or eight 16x16x4 FP32 MFMAs) consumes a whole
 // chunk: Re(P conj A) = sum_k P_k A_k.  Per chunk:
 //   * B (projections): LDS-DMA from the pre-encoded projection buffer;
-//   * A (translated, weighted images): each thread produces 8 consecutive
-//     pixels of one row -- phase factor from an exactly reduced argument,
-//     times corr*X, encoded -- and stores whole 16-byte segments to LDS;
+//   * A (translated, weighted images): each thread produces 8 (or 4)
+//     consecutive pixels of one row -- phase factor from an exactly reduced
+//     argument, times corr*X, encoded -- and stores them to LDS;
 //   * every wave reads its row and column fragments from LDS and issues the
 //     MFMAs.  A for chunk k+1 and the B copy of chunk k+1 are produced while
 //     the MFMAs of chunk k run; both buffers are double-buffered, one
 //     barrier per chunk.
+// Rows are packed: row R is translation R of the batch's CSR translation
+// arrays, so particles are not padded to a multiple of 16 rows.
 // Row/column r of a buffer holds KB bytes (128: FP32/BF16X3, 64: FP16) as
 // 16-byte segments; segment s sits at position s ^ ((r >> FSH) & (SEGS-1)),
 // which spreads the 16 rows read by one k-group of a wave over all banks.
@@ -140,12 +142,11 @@ hipError_t launch_project_wide(int form, const ProjView *vols, const float *bsca
 }
 
 // ---------------------------------------------------------------------------
-// A generation: 8 consecutive pixels of one row, written as whole segments
+// A generation: 8 or 4 consecutive pixels of one row per thread
 // ---------------------------------------------------------------------------
-// Rows past a particle's last translation (and rows of absent groups) keep
-// finite values -- zero shift, the particle's own image -- and need no
-// masking: row t of the product depends on row t of A only, and the
-// epilogue never stores or ranks such rows.
+// Rows past the batch's last translation keep finite values -- zero shift,
+// particle 0's image -- and need no masking: row R of the product depends on
+// row R of A only, and the epilogue never stores or ranks such rows.
 struct ARow {
     const float2 *xw;      // this row's particle: corr * X, padded to kpad
     double txr, tyr;       // translation in revolutions per pixel
