@@ -9,8 +9,9 @@
 // y - imgY when y >= imgY - maxR, else parked at x = maxR (diff2.cuh:593-617).
 //
 // One workgroup per job; each thread projects its pixels once per group of
-// up to 16 translations and keeps one float accumulator per translation; the
-// workgroup sums them in float64 in a fixed order.
+// up to 16 translations and keeps one float accumulator per translation (only
+// the job's own translations are scored: jobs of 4 do a quarter of the phase
+// work of jobs of 16); the workgroup sums them in float64 in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "device_common.hpp"
@@ -63,21 +64,25 @@ __global__ __launch_bounds__(kFT) void fine_kernel(const FineArgs a)
             const float xr = a.real[p], xi = a.imag[p], hw = 0.5f * a.corr[p];
 #pragma unroll
             for (int j = 0; j < kFChunk; ++j) {
-                float s, c;
-                phase_factor(fx, fy, txr[j], tyr[j], s, c);
-                const float dr = P.x - (c * xr - s * xi);
-                const float di = P.y - (c * xi + s * xr);
-                acc[j] += (dr * dr + di * di) * hw;
+                if (j < cn) {     // uniform over the workgroup
+                    float s, c;
+                    phase_factor(fx, fy, txr[j], tyr[j], s, c);
+                    const float dr = P.x - (c * xr - s * xi);
+                    const float di = P.y - (c * xi + s * xr);
+                    acc[j] += (dr * dr + di * di) * hw;
+                }
             }
         }
         // fixed-order block sums in float64: wave butterflies, then the waves
         const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
         for (int j = 0; j < kFChunk; ++j) {
-            double v = (double)acc[j];
+            if (j < cn) {
+                double v = (double)acc[j];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-            if (lane == 0) s_red[j][wave] = v;
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane == 0) s_red[j][wave] = v;
+            }
         }
         __syncthreads();
         if (tid < cn) {

@@ -372,7 +372,9 @@ def coarse_weights(trans_offset, d_trans_offset, n_classes, orientation_num, dif
     """rc_coarse_weights on torch device tensors; returns a dict with
     ``weights`` (float32, diff2 layout), ``sum`` (float64 [P]),
     ``significant_weight`` (float32 [P]), ``n_significant`` (int32 [P]),
-    ``mask`` (uint8, diff2 layout, or None) and ``max_log_weight``."""
+    ``mask`` (uint8, diff2 layout, or None) and ``max_log_weight``.  ``mask``
+    may be False (no mask), True (allocated here) or a uint8 device tensor of
+    at least ``diff2.numel()`` bytes."""
     import torch
     L = load()
     P = len(trans_offset) - 1
@@ -383,7 +385,8 @@ def coarse_weights(trans_offset, d_trans_offset, n_classes, orientation_num, dif
                sum=torch.empty(P, dtype=torch.float64, device=dev),
                significant_weight=torch.empty(P, dtype=torch.float32, device=dev),
                n_significant=torch.empty(P, dtype=torch.int32, device=dev),
-               mask=torch.empty(diff2.numel(), dtype=torch.uint8, device=dev) if mask else None,
+               mask=(mask if isinstance(mask, torch.Tensor) else
+                     torch.empty(diff2.numel(), dtype=torch.uint8, device=dev) if mask else None),
                max_log_weight=torch.empty(P, dtype=torch.float64, device=dev))
     off = np.ascontiguousarray(trans_offset, np.int32)
     d = rc_weights_desc()
@@ -400,7 +403,7 @@ def coarse_weights(trans_offset, d_trans_offset, n_classes, orientation_num, dif
     d.d_sum_weight = out["sum"].data_ptr()
     d.d_significant_weight = out["significant_weight"].data_ptr()
     d.d_n_significant = out["n_significant"].data_ptr()
-    d.d_significant = out["mask"].data_ptr() if mask else None
+    d.d_significant = out["mask"].data_ptr() if out["mask"] is not None else None
     d.d_max_log_weight = out["max_log_weight"].data_ptr()
     s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
     check(L.rc_coarse_weights(C.byref(d), s))

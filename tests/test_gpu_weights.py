@@ -28,16 +28,18 @@ def _priors(prob, rng, excluded=0.05):
 
 
 def _check(prob, t, out, fraction, lpo, lpt):
-    C, O = prob.n_classes, prob.orientation_num
-    off = np.asarray(prob.trans_offset, np.int64)
-    d = t["diff2"].cpu().numpy()
-    best = t["best_diff2"].cpu().numpy()
+    _check_arrays(t["diff2"].cpu().numpy(), t["best_diff2"].cpu().numpy(),
+                  np.asarray(prob.trans_offset, np.int64), prob.n_classes,
+                  prob.orientation_num, out, fraction, lpo, lpt)
+
+
+def _check_arrays(d, best, off, C, O, out, fraction, lpo, lpt):
     w = out["weights"].cpu().numpy()
     mask = out["mask"].cpu().numpy()
     S = out["sum"].cpu().numpy()
     ws = out["significant_weight"].cpu().numpy()
     ns = out["n_significant"].cpu().numpy()
-    for i in range(prob.n_particles):
+    for i in range(len(off) - 1):
         T = int(off[i + 1] - off[i])
         a, b = C * O * off[i], C * O * off[i + 1]
         pt = None if lpt is None else lpt[C * off[i]:C * off[i + 1]]
@@ -126,3 +128,56 @@ def test_weights_rejects_bad_fraction(lib, torch_cuda):
     m = torch.zeros(1, dtype=torch.float64, device="cuda")
     with pytest.raises(lib.RcError):
         lib.coarse_weights(off, torch.as_tensor(off).cuda(), 1, 2, x, m, fraction=0.0)
+
+
+def test_weights_crowded_buckets_and_alignment(lib, torch_cuda):
+    """Particles whose selected first-level bucket (exponent and three leading
+    significand bits) holds more samples than the kernel finishes in LDS --
+    near-equal and equal weights -- beside ones that fit; sizes that put the
+    particle boundaries inside 16-byte lines; weights and mask buffers at
+    another alignment than diff2.  The outputs match those of aligned buffers
+    bit for bit, and a run without a mask gives the same threshold and count."""
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    C, O = 1, 96
+    Ts = [70, 53, 80, 1, 45, 3]
+    off = np.concatenate([[0], np.cumsum(Ts)]).astype(np.int64)
+    parts, best = [], []
+    for i, T in enumerate(Ts):
+        n = C * O * T
+        base = 1000.0 + 10 * i
+        if i == 0:
+            x = base + rng.random(n) * 0.01            # weights in [0.99, 1]
+        elif i == 1:
+            x = np.full(n, base)                        # every weight 1
+        elif i == 2:                                    # 70 % crowded, rest spread
+            x = base + np.where(rng.random(n) < 0.7, rng.random(n) * 0.02, rng.random(n) * 50)
+        elif i == 3:
+            x = base + rng.random(n) * 3
+        else:
+            x = base + rng.random(n) * 200
+        x = x.astype(np.float32)
+        parts.append(x)
+        best.append(float(x.min()))
+    d = np.concatenate(parts)
+    best = np.asarray(best)
+    d_t = torch.as_tensor(d).cuda()
+    best_t = torch.as_tensor(best).cuda()
+    off32 = off.astype(np.int32)
+    doff = torch.as_tensor(off32).cuda()
+    n = d.size
+    for fraction in (0.999, 0.5):
+        ref = lib.coarse_weights(off32, doff, C, O, d_t, best_t, fraction=fraction)
+        wbuf = torch.full((n + 1,), -1.0, device="cuda")
+        mbuf = torch.full((n + 3,), 7, dtype=torch.uint8, device="cuda")
+        odd = lib.coarse_weights(off32, doff, C, O, d_t, best_t, fraction=fraction,
+                                 weights=wbuf[1:], mask=mbuf[3:])
+        nomask = lib.coarse_weights(off32, doff, C, O, d_t, best_t, fraction=fraction,
+                                    mask=False)
+        torch.cuda.synchronize()
+        _check_arrays(d, best, off, C, O, ref, fraction, None, None)
+        for k in ("weights", "sum", "significant_weight", "n_significant", "mask"):
+            assert torch.equal(ref[k], odd[k]), (fraction, k)
+        for k in ("weights", "sum", "significant_weight", "n_significant"):
+            assert torch.equal(ref[k], nomask[k]), (fraction, k)
+        assert wbuf[0].item() == -1.0 and mbuf[:3].tolist() == [7, 7, 7]
