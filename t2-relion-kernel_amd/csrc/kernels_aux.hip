@@ -210,8 +210,10 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const bool in = k0 + lk + 4 * u < kpad;
-            ra[u] = (a_ok && in) ? *reinterpret_cast<const float4 *>(ap + k0 + 4 * u) : zero;
-            rb[u] = (b_ok && in) ? *reinterpret_cast<const float4 *>(bp + k0 + 4 * u) : zero;
+            ra[u] = zero;
+            rb[u] = zero;
+            if (a_ok && in) ra[u] = *reinterpret_cast<const float4 *>(ap + k0 + 4 * u);
+            if (b_ok && in) rb[u] = *reinterpret_cast<const float4 *>(bp + k0 + 4 * u);
         }
     };
     floatx4 acc[2][2];
