@@ -503,6 +503,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             WideFrag<FORM> fb_reg[2];
             wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
             float2 v[PPT];
+            // raised issue priority while this wave has MFMAs to issue: the
+            // SIMD's other wave, past its MFMAs, yields to it (2-3 % here)
+            __builtin_amdgcn_s_setprio(1);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int fb = 0; fb < FN; ++fb) {
@@ -516,6 +519,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                     if ((j * FN) / PPT == fb) v[j] = a_pixel<PPT>(ar, xw, cxy, j);
                 __builtin_amdgcn_sched_barrier(0);
             }
+            __builtin_amdgcn_s_setprio(0);
             a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
             load_src(min(kn + 1, nrun - 1), xw, cxy);
         } else if (lead) {
