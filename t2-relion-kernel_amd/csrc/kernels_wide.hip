@@ -172,6 +172,35 @@ __device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[PPT
     return a_value(ar, x, y, ww);
 }
 
+// BF16x3 words of one value: hi = (bf16 re, bf16 im), lo = the remainders.
+__device__ __forceinline__ void encode_bf16x3(float2 z, uint32_t &hi, uint32_t &lo)
+{
+    __bf16 hr, lr, hm, lm;
+    split_bf16(z.x, hr, lr);
+    split_bf16(z.y, hm, lm);
+    hi = (uint32_t)__builtin_bit_cast(uint16_t, hr) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, hm) << 16);
+    lo = (uint32_t)__builtin_bit_cast(uint16_t, lr) |
+         ((uint32_t)__builtin_bit_cast(uint16_t, lm) << 16);
+}
+
+// Stores the BF16x3 words of pixels PPT*h .. PPT*h+PPT-1 into row r:
+// group g of 4 pixels goes to hi segment g and lo segment 4 + g.
+template <int PPT>
+__device__ __forceinline__ void a_store_words(const uint32_t (&hi)[PPT], const uint32_t (&lo)[PPT],
+                                              int h, int r, unsigned char *dst)
+{
+    constexpr int G = PPT / 4;
+#pragma unroll
+    for (int half = 0; half < G; ++half) {
+        const int g = G * h + half;
+        *reinterpret_cast<uint4 *>(dst + wide_swz<kBF16X3>(r, g) * 16) =
+            make_uint4(hi[4 * half], hi[4 * half + 1], hi[4 * half + 2], hi[4 * half + 3]);
+        *reinterpret_cast<uint4 *>(dst + wide_swz<kBF16X3>(r, 4 + g) * 16) =
+            make_uint4(lo[4 * half], lo[4 * half + 1], lo[4 * half + 2], lo[4 * half + 3]);
+    }
+}
+
 // Encodes the thread's PPT (8 or 4) values -- pixels PPT*h .. PPT*h+PPT-1 of
 // the chunk -- into LDS row buffer `dst` of row r.
 template <int FORM, int PPT>
@@ -181,27 +210,10 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], 
     static_assert(PPT == 8 || PPT == 4, "8 or 4 pixels per thread");
     constexpr int G = PPT / 4;   // 4-pixel groups (one 16-byte segment each)
     if constexpr (FORM == kBF16X3) {
-        // group g: hi segment g, lo segment 4 + g; 4 pixels (re, im) each
+        uint32_t hi[PPT], lo[PPT];
 #pragma unroll
-        for (int half = 0; half < G; ++half) {
-            uint32_t hi[4], lo[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float2 z = v[4 * half + i];
-                __bf16 hr, lr, hm, lm;
-                split_bf16(z.x, hr, lr);
-                split_bf16(z.y, hm, lm);
-                hi[i] = (uint32_t)__builtin_bit_cast(uint16_t, hr) |
-                        ((uint32_t)__builtin_bit_cast(uint16_t, hm) << 16);
-                lo[i] = (uint32_t)__builtin_bit_cast(uint16_t, lr) |
-                        ((uint32_t)__builtin_bit_cast(uint16_t, lm) << 16);
-            }
-            const int g = G * h + half;
-            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, g) * 16) =
-                make_uint4(hi[0], hi[1], hi[2], hi[3]);
-            *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, 4 + g) * 16) =
-                make_uint4(lo[0], lo[1], lo[2], lo[3]);
-        }
+        for (int j = 0; j < PPT; ++j) encode_bf16x3(v[j], hi[j], lo[j]);
+        a_store_words<PPT>(hi, lo, h, r, dst);
     } else if constexpr (FORM == kFP16) {
 #pragma unroll
         for (int half = 0; half < G; ++half) {
@@ -487,11 +499,13 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
         // the last step fetches its own chunk again into the idle buffers,
         // which keeps each path through the loop body one straight block
         const int kn = min(kc + 1, nrun - 1);
-        issue_B(kn, buf ^ 1);
+        if (STAGGER != 4) issue_B(kn, buf ^ 1);
         if (STAGGER == 4) {
             // one column slice of MFMAs and one pixel of the next chunk's A
             // per fenced segment; the image data for the next build were
-            // loaded at the end of the previous step
+            // loaded in the last segment of the previous step.  The fragment
+            // reads go first, so that the LDS serves them before the next
+            // chunk's copies land.
             const unsigned char *pa = sA[buf];
             const unsigned char *pb = sB[buf];
             WideFrag<FORM> fa_reg[FM];
@@ -502,7 +516,12 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             }
             WideFrag<FORM> fb_reg[2];
             wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
+            issue_B(kn, buf ^ 1);
             float2 v[PPT];
+            uint32_t hw[PPT], lw[PPT];   // BF16x3, 4 pixels: each encoded in its own segment
+            // pixel j goes to segment j * FN / PPT + kPixOff: the segments
+            // that wait for the first fragments carry no pixel work
+            constexpr int kPixOff = FN > PPT ? FN / PPT - 1 : 0;
             // raised issue priority while this wave has MFMAs to issue: the
             // SIMD's other wave, past its MFMAs, yields to it (2-3 % here)
             __builtin_amdgcn_s_setprio(1);
@@ -516,12 +535,22 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                 wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg[fb & 1]);
 #pragma unroll
                 for (int j = 0; j < PPT; ++j)
-                    if ((j * FN) / PPT == fb) v[j] = a_pixel<PPT>(ar, xw, cxy, j);
+                    if ((j * FN) / PPT + kPixOff == fb) {
+                        v[j] = a_pixel<PPT>(ar, xw, cxy, j);
+                        if constexpr (FORM == kBF16X3 && PPT == 4)
+                            encode_bf16x3(v[j], hw[j], lw[j]);
+                    }
+                // the next build's image data, once this step's last pixel is
+                // done (8-pixel threads: after the segments, for registers)
+                if (PPT == 4 && fb == FN - 1) load_src(min(kn + 1, nrun - 1), xw, cxy);
                 __builtin_amdgcn_sched_barrier(0);
             }
             __builtin_amdgcn_s_setprio(0);
-            a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
-            load_src(min(kn + 1, nrun - 1), xw, cxy);
+            if constexpr (FORM == kBF16X3 && PPT == 4)
+                a_store_words<PPT>(hw, lw, gh, gr, &sA[buf ^ 1][gr * KB]);
+            else
+                a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
+            if (PPT != 4) load_src(min(kn + 1, nrun - 1), xw, cxy);
         } else if (lead) {
             load_src(kn, xw, cxy);
             mfma_chunk(buf);
