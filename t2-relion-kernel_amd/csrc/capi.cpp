@@ -305,7 +305,10 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     p->xnorm = c.take<double>(p->P);
     p->half_corr = c.take<float>((size_t)p->P * p->kpad);
     p->ascale = c.take<float>(p->P);
-    p->proj = c.take<unsigned char>((size_t)p->C * p->O * p->nkc * chunk_bytes(p->form));
+    // padding: the wide kernels' copies of a last, partial column tile run
+    // up to one tile (512 columns x 128 bytes) past the last projection
+    p->proj = c.take<unsigned char>((size_t)p->C * p->O * p->nkc * chunk_bytes(p->form) +
+                                    kProjPad);
     p->pabs2 = c.take<float>((size_t)p->C * p->O * p->kpad);
     p->pnorm = c.take<float>((size_t)p->P * p->C * p->O);
     const size_t prows = p->packed ? (size_t)p->n_rows : (size_t)p->n_groups;

@@ -440,15 +440,21 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     // projections: [class][chunk][orientation][KB]; the chunk of a column
     // tile is one contiguous run of TN * KB bytes
     const unsigned char *cls_proj = a.proj + (size_t)cls * nkc * O * KB;
+    // Copy `it` of a thread fetches 16 bytes of column it * CPI + tid / SEGS:
+    // the swizzled segment does not depend on `it` (CPI >> kShift is a
+    // multiple of SEGS), so each copy is a uniform base plus one per-thread
+    // 32-bit offset.  Columns past the last orientation read the padding
+    // after the projections (or the next chunk's first columns): never stored.
+    constexpr int CPI = NT / SEGS;   // columns per copy instruction
+    static_assert(NT % SEGS == 0 && ((CPI >> WideEnc<FORM>::kShift) % SEGS) == 0,
+                  "copy offsets must repeat across copies");
+    const uint32_t b_off =
+        (uint32_t)((tid / SEGS) * KB + wide_swz<FORM>(tid / SEGS, tid % SEGS) * 16);
     auto issue_B = [&](int kc, int buf) {
-        const unsigned char *chunk = cls_proj + (size_t)kc * O * KB;
+        const unsigned char *chunk = cls_proj + ((size_t)kc * O + o0) * KB;
 #pragma unroll
         for (int it = 0; it < LPT; ++it) {
-            const int u = it * NT + tid;
-            const int col = u / SEGS, pos = u % SEGS;
-            const int seg = wide_swz<FORM>(col, pos);   // the swizzle is an involution
-            const int o = min(o0 + col, O - 1);
-            const unsigned char *src = chunk + (size_t)o * KB + seg * 16;
+            const unsigned char *src = chunk + (size_t)it * CPI * KB + b_off;
             unsigned char *dst = &sB[buf][(it * NT + wave * 64) * 16];
             __builtin_amdgcn_global_load_lds((gbl_void_w *)src, (lds_void_w *)dst, 16, 0, 0);
         }

@@ -50,6 +50,7 @@ struct ProjView {
 // Limits of this build (checked on the host before any launch).
 constexpr int kMaxPix = 1 << 20;          // pixels per image
 constexpr int kMaxTrans = 1 << 16;        // translations per particle
+constexpr size_t kProjPad = 512 * 128;    // bytes after the projections: one wide column tile
 constexpr int kChunk = 32;                // pixels per MFMA k-chunk
 constexpr int kGroupPix = 8;              // pixels per lane per chunk
 
