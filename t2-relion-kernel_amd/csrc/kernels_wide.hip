@@ -336,20 +336,22 @@ __device__ __forceinline__ void wide_mfma(floatx4 (&acc)[FM], const WideFrag<FOR
     }
 }
 
-// Wait for every vector-memory operation but the newest 2 * (PPT / 2) (the
-// next chunk's source loads, issued last).
-template <int PPT>
-__device__ __forceinline__ void wait_vmcnt_keep()
-{
-    if constexpr (PPT == 8)
-        asm volatile("s_waitcnt vmcnt(8)" ::: "memory");
-    else
-        asm volatile("s_waitcnt vmcnt(4)" ::: "memory");
-}
-
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));   // dword-aligned 16 B
 typedef __attribute__((address_space(3))) void lds_void_w;
 typedef __attribute__((address_space(1))) const void gbl_void_w;
+
+// 16-byte copy global -> LDS, lane l writing LDS byte address lds_addr + 16 l
+// (lds_addr uniform over the wave).  Written as asm so that the compiler does not see an LDS store: it
+// would otherwise make every later LDS read of the other buffer wait for all
+// copies.  The caller waits (vmcnt(0)) before the data are read.
+__device__ __forceinline__ void lds_copy16(const void *src, unsigned lds_addr)
+{
+    const unsigned m0 = __builtin_amdgcn_readfirstlane(lds_addr);
+    asm volatile("s_mov_b32 m0, %1\n\ts_nop 1\n\tglobal_load_lds_dwordx4 %0, off"
+                 :
+                 : "v"(src), "s"(m0)
+                 : "memory", "m0");
+}
 
 template <int I, int N, typename F>
 __device__ __forceinline__ void static_for(F &&f)
@@ -450,13 +452,13 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                   "copy offsets must repeat across copies");
     const uint32_t b_off =
         (uint32_t)((tid / SEGS) * KB + wide_swz<FORM>(tid / SEGS, tid % SEGS) * 16);
+    const unsigned sB_lds = (unsigned)(size_t)(__attribute__((address_space(3))) void *)&sB[0][0];
     auto issue_B = [&](int kc, int buf) {
         const unsigned char *chunk = cls_proj + ((size_t)kc * O + o0) * KB;
 #pragma unroll
         for (int it = 0; it < LPT; ++it) {
             const unsigned char *src = chunk + (size_t)it * CPI * KB + b_off;
-            unsigned char *dst = &sB[buf][(it * NT + wave * 64) * 16];
-            __builtin_amdgcn_global_load_lds((gbl_void_w *)src, (lds_void_w *)dst, 16, 0, 0);
+            lds_copy16(src, sB_lds + buf * Cfg::kBufB + (it * NT + wave * 64) * 16);
         }
     };
     auto load_src = [&](int kc, float4 (&xw)[PPT / 2], float4 (&cxy)[PPT / 2]) {
@@ -509,9 +511,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
         if (STAGGER == 4) {
             // one column slice of MFMAs and one pixel of the next chunk's A
             // per fenced segment; the image data for the next build were
-            // loaded in the last segment of the previous step.  The fragment
-            // reads go first, so that the LDS serves them before the next
-            // chunk's copies land.
+            // loaded at the top of the previous step.  The fragment reads go
+            // first, so that the LDS serves them before the next chunk's
+            // copies land.
             const unsigned char *pa = sA[buf];
             const unsigned char *pb = sB[buf];
             WideFrag<FORM> fa_reg[FM];
@@ -522,7 +524,19 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             }
             WideFrag<FORM> fb_reg[2];
             wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
+            // the copies below are invisible to the compiler's wait counting:
+            // have it wait for this step's image registers (loaded in the last
+            // step) before they are issued, not after
+#pragma unroll
+            for (int u = 0; u < PPT / 2; ++u)
+                asm volatile("" : "+v"(xw[u].x), "+v"(xw[u].y), "+v"(xw[u].z), "+v"(xw[u].w),
+                             "+v"(cxy[u].x), "+v"(cxy[u].y), "+v"(cxy[u].z), "+v"(cxy[u].w));
             issue_B(kn, buf ^ 1);
+            // 4-pixel threads load the image data of the step after this one
+            // a whole step early, into a second register set (8-pixel
+            // threads have no registers to spare: after the segments)
+            float4 xw2[PPT / 2], cxy2[PPT / 2];
+            if constexpr (PPT == 4) load_src(min(kn + 1, nrun - 1), xw2, cxy2);
             float2 v[PPT];
             uint32_t hw[PPT], lw[PPT];   // BF16x3, 4 pixels: each encoded in its own segment
             // pixel j goes to segment j * FN / PPT + kPixOff: the segments
@@ -546,9 +560,6 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                         if constexpr (FORM == kBF16X3 && PPT == 4)
                             encode_bf16x3(v[j], hw[j], lw[j]);
                     }
-                // the next build's image data, once this step's last pixel is
-                // done (8-pixel threads: after the segments, for registers)
-                if (PPT == 4 && fb == FN - 1) load_src(min(kn + 1, nrun - 1), xw, cxy);
                 __builtin_amdgcn_sched_barrier(0);
             }
             __builtin_amdgcn_s_setprio(0);
@@ -556,7 +567,15 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                 a_store_words<PPT>(hw, lw, gh, gr, &sA[buf ^ 1][gr * KB]);
             else
                 a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
-            if (PPT != 4) load_src(min(kn + 1, nrun - 1), xw, cxy);
+            if constexpr (PPT == 4) {
+#pragma unroll
+                for (int u = 0; u < PPT / 2; ++u) {
+                    xw[u] = xw2[u];
+                    cxy[u] = cxy2[u];
+                }
+            } else {
+                load_src(min(kn + 1, nrun - 1), xw, cxy);
+            }
         } else if (lead) {
             load_src(kn, xw, cxy);
             mfma_chunk(buf);
@@ -566,10 +585,8 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             load_src(min(kn + 1, nrun - 1), xw, cxy);
             mfma_chunk(buf);
         }
-        if (STAGGER == 4)
-            wait_vmcnt_keep<PPT>();   // the chunk's B copies; the next build's loads may fly
-        else
-            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        // the next chunk's B copies (and the image loads) land before the barrier
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
