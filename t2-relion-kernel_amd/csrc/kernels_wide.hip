@@ -340,17 +340,25 @@ typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));   // dword-al
 typedef __attribute__((address_space(3))) void lds_void_w;
 typedef __attribute__((address_space(1))) const void gbl_void_w;
 
-// 16-byte copy global -> LDS, lane l writing LDS byte address lds_addr + 16 l
-// (lds_addr uniform over the wave).  Written as asm so that the compiler does not see an LDS store: it
+// 16-byte copy global -> LDS: lane l reads base + off (base uniform, off its
+// own 32-bit offset) and writes LDS byte address lds_addr + 16 l (lds_addr
+// uniform over the wave).  Written as asm so that the compiler does not see an LDS store: it
 // would otherwise make every later LDS read of the other buffer wait for all
 // copies.  The caller waits (vmcnt(0)) before the data are read.
-__device__ __forceinline__ void lds_copy16(const void *src, unsigned lds_addr)
+__device__ __forceinline__ void lds_copy16(unsigned long long base, unsigned off,
+                                           unsigned lds_addr)
 {
-    const unsigned m0 = __builtin_amdgcn_readfirstlane(lds_addr);
-    asm volatile("s_mov_b32 m0, %1\n\ts_nop 1\n\tglobal_load_lds_dwordx4 %0, off"
+    asm volatile("s_mov_b32 m0, %2\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, %1"
                  :
-                 : "v"(src), "s"(m0)
+                 : "v"(off), "s"(base), "s"(lds_addr)
                  : "memory", "m0");
+}
+
+// A wave-uniform value, marked as such (so that it is kept in SGPRs).
+__device__ __forceinline__ unsigned uniform32(unsigned v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ unsigned long long uniform64(unsigned long long v)
+{
+    return ((unsigned long long)uniform32((unsigned)(v >> 32)) << 32) | uniform32((unsigned)v);
 }
 
 template <int I, int N, typename F>
@@ -452,14 +460,17 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                   "copy offsets must repeat across copies");
     const uint32_t b_off =
         (uint32_t)((tid / SEGS) * KB + wide_swz<FORM>(tid / SEGS, tid % SEGS) * 16);
-    const unsigned sB_lds = (unsigned)(size_t)(__attribute__((address_space(3))) void *)&sB[0][0];
+    // uniform parts of the copies' addresses, in SGPRs
+    const unsigned sB_lds = uniform32(
+        (unsigned)(size_t)(__attribute__((address_space(3))) void *)&sB[0][0] + wave * 64 * 16);
+    const unsigned long long b_base = uniform64((unsigned long long)(size_t)(cls_proj + (size_t)o0 * KB));
+    const unsigned b_stride = uniform32((unsigned)(O * KB));   // bytes per chunk of one class
     auto issue_B = [&](int kc, int buf) {
-        const unsigned char *chunk = cls_proj + ((size_t)kc * O + o0) * KB;
+        const unsigned long long chunk = b_base + (unsigned long long)(unsigned)kc * b_stride;
 #pragma unroll
-        for (int it = 0; it < LPT; ++it) {
-            const unsigned char *src = chunk + (size_t)it * CPI * KB + b_off;
-            lds_copy16(src, sB_lds + buf * Cfg::kBufB + (it * NT + wave * 64) * 16);
-        }
+        for (int it = 0; it < LPT; ++it)
+            lds_copy16(chunk + (unsigned long long)it * CPI * KB, b_off,
+                       sB_lds + buf * Cfg::kBufB + it * NT * 16);
     };
     auto load_src = [&](int kc, float4 (&xw)[PPT / 2], float4 (&cxy)[PPT / 2]) {
         const float4 *xp = reinterpret_cast<const float4 *>(ar.xw + kc * 16);
