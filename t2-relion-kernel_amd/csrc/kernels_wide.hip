@@ -610,30 +610,51 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     // increasing orientation, so a strict "<" keeps the lowest flat index
     // among equal minima of a row.
     const int C = a.n_classes;
+    // Global loads cannot move past the stores (the compiler must assume
+    // they alias), so every load of the common case -- the four rows of a
+    // fragment row in one particle -- is issued ahead of the stores, for a
+    // group of fragment rows, in two dependent rounds; the group's stores
+    // then follow without waits.
+    // (Two fragment rows at a time: all four would need more registers.)
+    constexpr int EG = (FM >= 2 && FORM != kFP16) ? 2 : 1;   // fragment rows per load group
+    int e_part[EG], e_b0[EG], e_T[EG];
+    bool e_same[EG];
+    double e_xn[EG];
+    float e_inv[FORM == kFP16 ? EG : 1];
+    float e_pn[EG][FN];
+    auto epilogue_loads = [&](auto g_c) {
+      static_for<0, EG>([&](auto e_c) {
+        constexpr int e = decltype(e_c)::value;
+        constexpr int fa = decltype(g_c)::value * EG + e;
+        const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
+        const int part = R0 < a.n_rows ? a.row_part[R0] : 0;
+        const int part3 = R0 + 3 < a.n_rows ? a.row_part[R0 + 3] : -1;
+        e_part[e] = part;
+        e_same[e] = part3 == part;   // rows are sorted by particle
+      });
+      static_for<0, EG>([&](auto e_c) {
+        constexpr int e = decltype(e_c)::value;
+        const int part = e_part[e];
+        e_b0[e] = a.trans_offset[part];
+        e_T[e] = a.trans_offset[part + 1];
+        e_xn[e] = a.xnorm[part];
+        if constexpr (FORM == kFP16) e_inv[e] = 1.f / (a.ascale[part] * a.bscale[cls]);
+#pragma unroll
+        for (int fb = 0; fb < FN; ++fb) {
+            const int o = min(o0 + rb0 + fb * 16 + r16, O - 1);
+            e_pn[e][fb] = a.pnorm[((size_t)part * C + cls) * O + o];
+        }
+      });
+    };
     // one call per fragment row, with fa a compile-time constant so that the
     // accumulators stay in registers
     auto epilogue_row = [&](auto fa_c) {
         constexpr int fa = decltype(fa_c)::value;
+        constexpr int e = fa % EG;
+        if constexpr (e == 0) epilogue_loads(std::integral_constant<int, fa / EG>{});
         const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
-        int rpart[4], rT[4], rt[4];
-        size_t rbase[4];
-        double rxn[4];
-        float rinv[4];
+        int rT[4], rt[4];
         bool rok[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int R = R0 + v;
-            rok[v] = R < a.n_rows;
-            const int part = rok[v] ? a.row_part[R] : 0;
-            const int b0 = a.trans_offset[part];
-            rpart[v] = part;
-            rT[v] = a.trans_offset[part + 1] - b0;
-            rt[v] = R - b0;
-            rbase[v] = (size_t)C * O * b0 + (size_t)cls * O * rT[v];
-            rxn[v] = a.xnorm[part];
-            rinv[v] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
-        }
-        const bool same = rok[3] && rpart[0] == rpart[3];   // rows are sorted by particle
         double best[4];
         int best_fb[4];
 #pragma unroll
@@ -641,57 +662,99 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             best[v] = DBL_MAX;
             best_fb[v] = -1;
         }
+        if (e_same[e]) {
+            // four consecutive translations of one particle: 16 contiguous bytes
+            const int b0 = e_b0[e], T = e_T[e] - b0;
 #pragma unroll
-        for (int fb = 0; fb < FN; ++fb) {
-            const int o = o0 + rb0 + fb * 16 + r16;
-            if (o >= O || !rok[0]) continue;
-            float out[4];
-            double pn[4];
-            if (same) {
-                const double p = (double)a.pnorm[((size_t)rpart[0] * C + cls) * O + o];
-                pn[0] = pn[1] = pn[2] = pn[3] = p;
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    pn[v] = rok[v] ? (double)a.pnorm[((size_t)rpart[v] * C + cls) * O + o] : 0.0;
+            for (int v = 0; v < 4; ++v) {
+                rok[v] = true;
+                rT[v] = T;
+                rt[v] = R0 + v - b0;
             }
-            float init[4] = {0.f, 0.f, 0.f, 0.f};
-            const size_t k0 = rbase[0] + (size_t)o * rT[0] + rt[0];
-            if (!a.cc && a.diff2_init) {
-                if (same) {
+            const size_t rbase = (size_t)C * O * b0 + (size_t)cls * O * T;
+#pragma unroll
+            for (int fb = 0; fb < FN; ++fb) {
+                const int o = o0 + rb0 + fb * 16 + r16;
+                if (o >= O) continue;
+                const size_t k0 = rbase + (size_t)o * T + rt[0];
+                float init[4] = {0.f, 0.f, 0.f, 0.f};
+                if (!a.cc && a.diff2_init) {
                     const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
                     init[0] = x.x;
                     init[1] = x.y;
                     init[2] = x.z;
                     init[3] = x.w;
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        if (rok[v]) init[v] = a.diff2_init[rbase[v] + (size_t)o * rT[v] + rt[v]];
                 }
+                const double pn = (double)e_pn[e][fb];
+                float out[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float cval = acc[fb][fa][v] * (FORM == kFP16 ? e_inv[FORM == kFP16 ? e : 0] : 1.f);
+                    double d;
+                    if (a.cc) {
+                        d = -(double)cval / sqrt(2.0 * pn);
+                    } else {
+                        d = pn + e_xn[e] - (double)cval;
+                        if (a.diff2_init) d += (double)init[v];
+                    }
+                    out[v] = (float)d;
+                    if (d < best[v]) {
+                        best[v] = d;
+                        best_fb[v] = fb;
+                    }
+                }
+                *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
             }
+        } else {
+            // a particle boundary (or the end of the batch) inside the four
+            // rows: row by row
+            int rpart[4];
+            size_t rbase[4];
+            double rxn[4];
+            float rinv[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float cval = acc[fb][fa][v] * rinv[v];
-                double d;
-                if (a.cc) {
-                    d = -(double)cval / sqrt(2.0 * pn[v]);
-                } else {
-                    d = pn[v] + rxn[v] - (double)cval;
-                    if (a.diff2_init) d += (double)init[v];
-                }
-                out[v] = (float)d;
-                if (rok[v] && d < best[v]) {
-                    best[v] = d;
-                    best_fb[v] = fb;
-                }
+                const int R = R0 + v;
+                rok[v] = R < a.n_rows;
+                const int part = rok[v] ? a.row_part[R] : 0;
+                const int b0 = a.trans_offset[part];
+                rpart[v] = part;
+                rT[v] = a.trans_offset[part + 1] - b0;
+                rt[v] = R - b0;
+                rbase[v] = (size_t)C * O * b0 + (size_t)cls * O * rT[v];
+                rxn[v] = a.xnorm[part];
+                rinv[v] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
             }
-            if (same) {
-                *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
-            } else {
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    if (rok[v]) a.diff2[rbase[v] + (size_t)o * rT[v] + rt[v]] = out[v];
+            for (int fb = 0; fb < FN; ++fb) {
+                const int o = o0 + rb0 + fb * 16 + r16;
+                if (o >= O || !rok[0]) continue;
+                double pn[4];
+                float init[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    pn[v] = rok[v] ? (double)a.pnorm[((size_t)rpart[v] * C + cls) * O + o] : 0.0;
+                    if (!a.cc && a.diff2_init && rok[v])
+                        init[v] = a.diff2_init[rbase[v] + (size_t)o * rT[v] + rt[v]];
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float cval = acc[fb][fa][v] * rinv[v];
+                    double d;
+                    if (a.cc) {
+                        d = -(double)cval / sqrt(2.0 * pn[v]);
+                    } else {
+                        d = pn[v] + rxn[v] - (double)cval;
+                        if (a.diff2_init) d += (double)init[v];
+                    }
+                    if (rok[v]) {
+                        a.diff2[rbase[v] + (size_t)o * rT[v] + rt[v]] = (float)d;
+                        if (d < best[v]) {
+                            best[v] = d;
+                            best_fb[v] = fb;
+                        }
+                    }
+                }
             }
         }
         if (a.part_min) {
