@@ -35,7 +35,7 @@ def time_form(lib, torch, prob, projs, form, steps, warmup, tune=True):
     tuned = None
     if tune:
         probe = lib.plan_for_problem(prob, projs, form)
-        tuned = probe.autotune(t, iters=2)
+        tuned = probe.autotune(t, iters=5)
         probe.close()
         log(f"{form}: autotuner picked variant {tuned[0]} ({tuned[1]:.3f} ms cross term)")
     plan = lib.plan_for_problem(prob, projs, form)

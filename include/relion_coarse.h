@@ -192,8 +192,9 @@ int rc_plan_capture(rc_plan *plan, const rc_batch_desc *desc, void *stream);
 int rc_plan_replay(rc_plan *plan, void *stream);
 
 /* Autotuner: time every cross-term variant of desc's form on desc's shape
- * (one plan per variant in turn, one warm-up and `iters` timed launches,
- * HIP events on `stream`), keep the fastest in a process-wide table keyed
+ * (one plan per variant in turn, two warm-up and `iters` timed launches,
+ * HIP events on `stream`, the median compared), keep the fastest in a
+ * process-wide table keyed
  * by the shape (particles, classes, orientations, pixels, form, row groups)
  * that later rc_plan_create calls consult, and return it.  Allocates and
  * synchronises the stream; not for graph capture. */

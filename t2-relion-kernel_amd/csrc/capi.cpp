@@ -729,7 +729,8 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
         rc_plan *p = nullptr;
         int st = rc_plan_create(d, &p);
         if (st == RC_OK) st = rc_plan_timing(p, iters);
-        if (st == RC_OK) st = rc_plan_execute(p, d, stream);
+        // two untimed launches first
+        for (int k = 0; st == RC_OK && k < 2; ++k) st = rc_plan_execute(p, d, stream);
         if (st == RC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
             set_error("stream synchronisation failed while tuning");
             st = RC_ERR_DEVICE;
@@ -743,9 +744,11 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
             g_variant_override = saved;
             return st;
         }
-        float t = 0.f;
-        for (int k = 0; k < got; ++k) t += ms[k];
-        t /= std::max(got, 1);
+        // median: robust to the odd slow launch (clock ramps, other streams)
+        std::sort(ms.begin(), ms.begin() + got);
+        const float t = got == 0 ? 0.f
+                        : (got & 1) ? ms[got / 2]
+                                    : 0.5f * (ms[got / 2 - 1] + ms[got / 2]);
         if (best < 0 || t < best_t) {
             best = v;
             best_t = t;
