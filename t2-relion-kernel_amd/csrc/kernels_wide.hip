@@ -370,6 +370,25 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
+// Lane i of each 16-lane row receives lane (i + N) mod 16 of the same row.
+template <int N>
+__device__ __forceinline__ int ror16(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, 0x120 + N, 0xf, 0xf, false);   // row_ror:N
+}
+template <int N>
+__device__ __forceinline__ long long ror16(long long v)
+{
+    const int lo = ror16<N>((int)(unsigned long long)v);
+    const int hi = ror16<N>((int)((unsigned long long)v >> 32));
+    return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+template <int N>
+__device__ __forceinline__ double ror16(double v)
+{
+    return __builtin_bit_cast(double, ror16<N>(__builtin_bit_cast(long long, v)));
+}
+
 template <int FORM, int WM, int WN, int FM, int FN>
 struct WideCfg {
     static constexpr int kThreads = 64 * WM * WN;
@@ -767,15 +786,18 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                     const int o = o0 + rb0 + best_fb[v] * 16 + r16;
                     bk = (long long)(cls * O + o) * rT[v] + rt[v];
                 }
-#pragma unroll
-                for (int off = 8; off > 0; off >>= 1) {
-                    const double ob = __shfl_xor(bv, off, 64);
-                    const long long ok2 = __shfl_xor(bk, off, 64);
+                // rotations within each 16-lane row (DPP moves on the vector
+                // ALU): after 8, 4, 2, 1 every lane holds the row minimum
+                // (the lowest index among equal values)
+                static_for<0, 4>([&](auto st_c) {
+                    constexpr int rot = 8 >> decltype(st_c)::value;
+                    const double ob = ror16<rot>(bv);
+                    const long long ok2 = ror16<rot>(bk);
                     if (ob < bv || (ob == bv && ok2 < bk)) {
                         bv = ob;
                         bk = ok2;
                     }
-                }
+                });
                 if (r16 == 0 && rok[v]) {
                     const size_t u = ((size_t)(R0 + v) * n_ntiles + ntile) * WN + wn;
                     a.part_min[u] = bv;
