@@ -173,15 +173,18 @@ __device__ __forceinline__ float2 a_pixel(const ARow &ar, const float4 (&xw)[PPT
 }
 
 // BF16x3 words of one value: hi = (bf16 re, bf16 im), lo = the remainders.
+// (Two-lane vector conversions: one packed round-to-nearest-even conversion
+// per word, the same values as split_bf16 on each component.)
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void encode_bf16x3(float2 z, uint32_t &hi, uint32_t &lo)
 {
-    __bf16 hr, lr, hm, lm;
-    split_bf16(z.x, hr, lr);
-    split_bf16(z.y, hm, lm);
-    hi = (uint32_t)__builtin_bit_cast(uint16_t, hr) |
-         ((uint32_t)__builtin_bit_cast(uint16_t, hm) << 16);
-    lo = (uint32_t)__builtin_bit_cast(uint16_t, lr) |
-         ((uint32_t)__builtin_bit_cast(uint16_t, lm) << 16);
+    const f32x2 v = {z.x, z.y};
+    const bf16x2 h = __builtin_convertvector(v, bf16x2);
+    const f32x2 hv = __builtin_convertvector(h, f32x2);
+    const f32x2 r = {v.x - hv.x, v.y - hv.y};
+    hi = __builtin_bit_cast(uint32_t, h);
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
 }
 
 // Stores the BF16x3 words of pixels PPT*h .. PPT*h+PPT-1 into row r:
