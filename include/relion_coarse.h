@@ -193,7 +193,8 @@ int rc_plan_replay(rc_plan *plan, void *stream);
 
 /* Autotuner: time every cross-term variant of desc's form on desc's shape
  * (one plan per variant in turn, two warm-up and `iters` timed launches,
- * HIP events on `stream`, the median compared), keep the fastest in a
+ * HIP events on `stream`; two sweeps over the table, each variant's better
+ * median compared), keep the fastest in a
  * process-wide table keyed
  * by the shape (particles, classes, orientations, pixels, form, row groups)
  * that later rc_plan_create calls consult, and return it.  Allocates and

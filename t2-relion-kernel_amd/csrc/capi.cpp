@@ -724,7 +724,11 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
     int best = -1;
     float best_t = 0.f;
     std::vector<float> ms(iters);
-    for (int v = 0; v < n; ++v) {
+    // two sweeps over the table, each variant keeping its better median: a
+    // variant timed early is not penalised by the clock still ramping up
+    std::vector<float> vt(n, -1.f);
+    for (int sweep = 0; sweep < 2 * n; ++sweep) {
+        const int v = sweep % n;
         g_variant_override = v;
         rc_plan *p = nullptr;
         int st = rc_plan_create(d, &p);
@@ -749,11 +753,13 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
         const float t = got == 0 ? 0.f
                         : (got & 1) ? ms[got / 2]
                                     : 0.5f * (ms[got / 2 - 1] + ms[got / 2]);
-        if (best < 0 || t < best_t) {
-            best = v;
-            best_t = t;
-        }
+        if (vt[v] < 0.f || t < vt[v]) vt[v] = t;
     }
+    for (int v = 0; v < n; ++v)
+        if (best < 0 || vt[v] < best_t) {
+            best = v;
+            best_t = vt[v];
+        }
     g_variant_override = saved;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
