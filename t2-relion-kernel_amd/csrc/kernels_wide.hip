@@ -409,6 +409,11 @@ struct WideCfg {
 // SIMD holds keep its matrix core and its vector ALU busy at the same time.
 // STAGGER 4: each column slice's MFMAs share a fenced scheduling segment
 // with one pixel of the next chunk's A build.
+// epilogue output forms
+constexpr int kEpiPlain = 0;   // diff2 from the two norms and the cross term
+constexpr int kEpiInit = 1;    // the same, added to diff2_init
+constexpr int kEpiCC = 2;      // cross-correlation score
+
 template <int FORM, int WM, int WN, int FM, int FN, int STAGGER>
 __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArgs a)
 {
@@ -639,177 +644,203 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     // then follow without waits.
     // (Two fragment rows at a time: all four would need more registers.)
     constexpr int EG = (FM >= 2 && FORM != kFP16) ? 2 : 1;   // fragment rows per load group
-    int e_part[EG], e_b0[EG], e_T[EG];
-    bool e_same[EG];
-    double e_xn[EG];
-    float e_inv[FORM == kFP16 ? EG : 1];
-    float e_pn[EG][FN];
-    auto epilogue_loads = [&](auto g_c) {
-      static_for<0, EG>([&](auto e_c) {
-        constexpr int e = decltype(e_c)::value;
-        constexpr int fa = decltype(g_c)::value * EG + e;
-        const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
-        const int part = R0 < a.n_rows ? a.row_part[R0] : 0;
-        const int part3 = R0 + 3 < a.n_rows ? a.row_part[R0 + 3] : -1;
-        e_part[e] = part;
-        e_same[e] = part3 == part;   // rows are sorted by particle
-      });
-      static_for<0, EG>([&](auto e_c) {
-        constexpr int e = decltype(e_c)::value;
-        const int part = e_part[e];
-        e_b0[e] = a.trans_offset[part];
-        e_T[e] = a.trans_offset[part + 1];
-        e_xn[e] = a.xnorm[part];
-        if constexpr (FORM == kFP16) e_inv[e] = 1.f / (a.ascale[part] * a.bscale[cls]);
+    // The three output forms -- diff2, diff2 added to diff2_init, and the
+    // cross-correlation score -- are separate compile-time instances chosen
+    // once per tile.  A per-element test would put every form's arithmetic
+    // (the CC score's float64 division and square root among it) into the
+    // instruction stream of the common form; every wave fetches the whole
+    // epilogue once per tile, so its size is paid for on each tile.
+    auto epilogue = [&](auto mode_c) {
+        constexpr int MODE = decltype(mode_c)::value;
+        int e_part[EG], e_b0[EG], e_T[EG];
+        bool e_same[EG];
+        double e_xn[EG];
+        float e_inv[FORM == kFP16 ? EG : 1];
+        float e_pn[EG][FN];
+        auto epilogue_loads = [&](auto g_c) {
+          static_for<0, EG>([&](auto e_c) {
+            constexpr int e = decltype(e_c)::value;
+            constexpr int fa = decltype(g_c)::value * EG + e;
+            const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
+            const int part = R0 < a.n_rows ? a.row_part[R0] : 0;
+            const int part3 = R0 + 3 < a.n_rows ? a.row_part[R0 + 3] : -1;
+            e_part[e] = part;
+            e_same[e] = part3 == part;   // rows are sorted by particle
+          });
+          static_for<0, EG>([&](auto e_c) {
+            constexpr int e = decltype(e_c)::value;
+            const int part = e_part[e];
+            e_b0[e] = a.trans_offset[part];
+            e_T[e] = a.trans_offset[part + 1];
+            e_xn[e] = a.xnorm[part];
+            if constexpr (FORM == kFP16) e_inv[e] = 1.f / (a.ascale[part] * a.bscale[cls]);
 #pragma unroll
-        for (int fb = 0; fb < FN; ++fb) {
-            const int o = min(o0 + rb0 + fb * 16 + r16, O - 1);
-            e_pn[e][fb] = a.pnorm[((size_t)part * C + cls) * O + o];
-        }
-      });
-    };
-    // one call per fragment row, with fa a compile-time constant so that the
-    // accumulators stay in registers
-    auto epilogue_row = [&](auto fa_c) {
-        constexpr int fa = decltype(fa_c)::value;
-        constexpr int e = fa % EG;
-        if constexpr (e == 0) epilogue_loads(std::integral_constant<int, fa / EG>{});
-        const int R0 = mtile * TM + (wm * FM + fa) * 16 + 4 * q;
-        int rT[4], rt[4];
-        bool rok[4];
-        double best[4];
-        int best_fb[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            best[v] = DBL_MAX;
-            best_fb[v] = -1;
-        }
-        if (e_same[e]) {
-            // four consecutive translations of one particle: 16 contiguous bytes
-            const int b0 = e_b0[e], T = e_T[e] - b0;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                rok[v] = true;
-                rT[v] = T;
-                rt[v] = R0 + v - b0;
+            for (int fb = 0; fb < FN; ++fb) {
+                const int o = min(o0 + rb0 + fb * 16 + r16, O - 1);
+                e_pn[e][fb] = a.pnorm[((size_t)part * C + cls) * O + o];
             }
-            const size_t rbase = (size_t)C * O * b0 + (size_t)cls * O * T;
+          });
+        };
+        // one float64 score
+        auto score = [&](float cval, double pn, double xn, float init) -> double {
+            if constexpr (MODE == kEpiCC) {
+                return -(double)cval / sqrt(2.0 * pn);
+            } else {
+                double d = pn + xn - (double)cval;
+                if constexpr (MODE == kEpiInit) d += (double)init;
+                return d;
+            }
+        };
+        // one call per group of EG fragment rows, with the fragment rows
+        // compile-time constants so that the accumulators stay in registers
+        auto epilogue_group = [&](auto g_c) {
+            constexpr int g = decltype(g_c)::value;
+            epilogue_loads(g_c);
+            int R0[EG], rT[EG][4], rt[EG][4];
+            bool rok[EG][4];
+            double best[EG][4];
+            int best_fb[EG];   // byte v: the column slice of row v's minimum, 0xff for none
+            static_for<0, EG>([&](auto e_c) {
+                constexpr int e = decltype(e_c)::value;
+                R0[e] = mtile * TM + (wm * FM + g * EG + e) * 16 + 4 * q;
+                const int T = e_T[e] - e_b0[e];
+                best_fb[e] = -1;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    rok[e][v] = true;
+                    rT[e][v] = T;
+                    rt[e][v] = R0[e] + v - e_b0[e];
+                    best[e][v] = DBL_MAX;
+                }
+            });
+            auto set_fb = [](int &w, int v, int fb) { w = (w & ~(0xff << (8 * v))) | (fb << (8 * v)); };
+            // Common case: the lane's four rows are four consecutive
+            // translations of one particle, 16 contiguous output bytes.
+            // Orientation-major, so that the group's fragment rows of one
+            // orientation -- adjacent runs of one output row -- are stored
+            // back to back.
 #pragma unroll
             for (int fb = 0; fb < FN; ++fb) {
                 const int o = o0 + rb0 + fb * 16 + r16;
-                if (o >= O) continue;
-                const size_t k0 = rbase + (size_t)o * T + rt[0];
-                float init[4] = {0.f, 0.f, 0.f, 0.f};
-                if (!a.cc && a.diff2_init) {
-                    const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
-                    init[0] = x.x;
-                    init[1] = x.y;
-                    init[2] = x.z;
-                    init[3] = x.w;
-                }
-                const double pn = (double)e_pn[e][fb];
-                float out[4];
+                static_for<0, EG>([&](auto e_c) {
+                    constexpr int e = decltype(e_c)::value;
+                    constexpr int fa = g * EG + e;
+                    if (o < O && e_same[e]) {
+                        const int T = rT[e][0];
+                        const size_t k0 = (size_t)C * O * e_b0[e] + (size_t)cls * O * T +
+                                          (size_t)o * T + rt[e][0];
+                        float init[4] = {0.f, 0.f, 0.f, 0.f};
+                        if constexpr (MODE == kEpiInit) {
+                            const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
+                            init[0] = x.x;
+                            init[1] = x.y;
+                            init[2] = x.z;
+                            init[3] = x.w;
+                        }
+                        const double pn = (double)e_pn[e][fb];
+                        float out[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float cval = acc[fb][fa][v] * (FORM == kFP16 ? e_inv[FORM == kFP16 ? e : 0] : 1.f);
-                    double d;
-                    if (a.cc) {
-                        d = -(double)cval / sqrt(2.0 * pn);
-                    } else {
-                        d = pn + e_xn[e] - (double)cval;
-                        if (a.diff2_init) d += (double)init[v];
+                        for (int v = 0; v < 4; ++v) {
+                            const float cval =
+                                acc[fb][fa][v] * (FORM == kFP16 ? e_inv[FORM == kFP16 ? e : 0] : 1.f);
+                            const double d = score(cval, pn, e_xn[e], init[v]);
+                            out[v] = (float)d;
+                            if (d < best[e][v]) {
+                                best[e][v] = d;
+                                set_fb(best_fb[e], v, fb);
+                            }
+                        }
+                        *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
                     }
-                    out[v] = (float)d;
-                    if (d < best[v]) {
-                        best[v] = d;
-                        best_fb[v] = fb;
-                    }
-                }
-                *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
+                });
             }
-        } else {
-            // a particle boundary (or the end of the batch) inside the four
-            // rows: row by row
-            int rpart[4];
-            size_t rbase[4];
-            double rxn[4];
-            float rinv[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int R = R0 + v;
-                rok[v] = R < a.n_rows;
-                const int part = rok[v] ? a.row_part[R] : 0;
-                const int b0 = a.trans_offset[part];
-                rpart[v] = part;
-                rT[v] = a.trans_offset[part + 1] - b0;
-                rt[v] = R - b0;
-                rbase[v] = (size_t)C * O * b0 + (size_t)cls * O * rT[v];
-                rxn[v] = a.xnorm[part];
-                rinv[v] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
-            }
-#pragma unroll
-            for (int fb = 0; fb < FN; ++fb) {
-                const int o = o0 + rb0 + fb * 16 + r16;
-                if (o >= O || !rok[0]) continue;
-                double pn[4];
-                float init[4] = {0.f, 0.f, 0.f, 0.f};
+            // A particle boundary (or the end of the batch) inside the lane's
+            // four rows: row by row.
+            static_for<0, EG>([&](auto e_c) {
+                constexpr int e = decltype(e_c)::value;
+                constexpr int fa = g * EG + e;
+                if (e_same[e]) return;
+                int rpart[4];
+                size_t rb[4];
+                double rxn[4];
+                float rinv[4];
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    pn[v] = rok[v] ? (double)a.pnorm[((size_t)rpart[v] * C + cls) * O + o] : 0.0;
-                    if (!a.cc && a.diff2_init && rok[v])
-                        init[v] = a.diff2_init[rbase[v] + (size_t)o * rT[v] + rt[v]];
+                    const int R = R0[e] + v;
+                    rok[e][v] = R < a.n_rows;
+                    const int part = rok[e][v] ? a.row_part[R] : 0;
+                    const int b0 = a.trans_offset[part];
+                    rpart[v] = part;
+                    rT[e][v] = a.trans_offset[part + 1] - b0;
+                    rt[e][v] = R - b0;
+                    rb[v] = (size_t)C * O * b0 + (size_t)cls * O * rT[e][v];
+                    rxn[v] = a.xnorm[part];
+                    rinv[v] = (FORM == kFP16) ? 1.f / (a.ascale[part] * a.bscale[cls]) : 1.f;
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const float cval = acc[fb][fa][v] * rinv[v];
-                    double d;
-                    if (a.cc) {
-                        d = -(double)cval / sqrt(2.0 * pn[v]);
-                    } else {
-                        d = pn[v] + rxn[v] - (double)cval;
-                        if (a.diff2_init) d += (double)init[v];
+                for (int fb = 0; fb < FN; ++fb) {
+                    const int o = o0 + rb0 + fb * 16 + r16;
+                    if (o >= O || !rok[e][0]) continue;
+                    double pn[4];
+                    float init[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        pn[v] = rok[e][v] ? (double)a.pnorm[((size_t)rpart[v] * C + cls) * O + o] : 0.0;
+                        if (MODE == kEpiInit && rok[e][v])
+                            init[v] = a.diff2_init[rb[v] + (size_t)o * rT[e][v] + rt[e][v]];
                     }
-                    if (rok[v]) {
-                        a.diff2[rbase[v] + (size_t)o * rT[v] + rt[v]] = (float)d;
-                        if (d < best[v]) {
-                            best[v] = d;
-                            best_fb[v] = fb;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const double d = score(acc[fb][fa][v] * rinv[v], pn[v], rxn[v], init[v]);
+                        if (rok[e][v]) {
+                            a.diff2[rb[v] + (size_t)o * rT[e][v] + rt[e][v]] = (float)d;
+                            if (d < best[e][v]) {
+                                best[e][v] = d;
+                                set_fb(best_fb[e], v, fb);
+                            }
                         }
                     }
                 }
-            }
-        }
-        if (a.part_min) {
-            // per row: reduce over the 16 lanes that share q (the columns)
+            });
+            if (!a.part_min) return;
+            // Per row, over the 16 lanes that share q (the columns): the
+            // minimum value, then the lowest orientation among the lanes
+            // holding it -- within a row the flat index grows with the
+            // orientation, so this is the lowest index among equal minima.
+            // Rotations 8, 4, 2, 1 within each 16-lane row (DPP moves on the
+            // vector ALU) leave the result in every lane.
+            static_for<0, EG>([&](auto e_c) {
+                constexpr int e = decltype(e_c)::value;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                double bv = best[v];
-                long long bk = 0x7fffffffffffffffLL;
-                if (best_fb[v] >= 0) {
-                    const int o = o0 + rb0 + best_fb[v] * 16 + r16;
-                    bk = (long long)(cls * O + o) * rT[v] + rt[v];
-                }
-                // rotations within each 16-lane row (DPP moves on the vector
-                // ALU): after 8, 4, 2, 1 every lane holds the row minimum
-                // (the lowest index among equal values)
-                static_for<0, 4>([&](auto st_c) {
-                    constexpr int rot = 8 >> decltype(st_c)::value;
-                    const double ob = ror16<rot>(bv);
-                    const long long ok2 = ror16<rot>(bk);
-                    if (ob < bv || (ob == bv && ok2 < bk)) {
-                        bv = ob;
-                        bk = ok2;
+                for (int v = 0; v < 4; ++v) {
+                    double bv = best[e][v];
+                    static_for<0, 4>([&](auto st_c) {
+                        const double ob = ror16<(8 >> decltype(st_c)::value)>(bv);
+                        bv = ob < bv ? ob : bv;
+                    });
+                    const int bfb = (best_fb[e] >> (8 * v)) & 0xff;
+                    int bo = bfb != 0xff && best[e][v] == bv ? rb0 + bfb * 16 + r16 : 0x7fffffff;
+                    static_for<0, 4>([&](auto st_c) {
+                        bo = min(bo, ror16<(8 >> decltype(st_c)::value)>(bo));
+                    });
+                    if (r16 == 0 && rok[e][v]) {
+                        const size_t u = ((size_t)(R0[e] + v) * n_ntiles + ntile) * WN + wn;
+                        a.part_min[u] = bv;
+                        a.part_idx[u] = bo == 0x7fffffff
+                                            ? 0x7fffffffffffffffLL
+                                            : (long long)(cls * O + o0 + bo) * rT[e][v] + rt[e][v];
                     }
-                });
-                if (r16 == 0 && rok[v]) {
-                    const size_t u = ((size_t)(R0 + v) * n_ntiles + ntile) * WN + wn;
-                    a.part_min[u] = bv;
-                    a.part_idx[u] = bk;
                 }
-            }
-        }
+            });
+        };
+        static_for<0, FM / EG>(epilogue_group);
     };
-    static_for<0, FM>(epilogue_row);
+    if (a.cc)
+        epilogue(std::integral_constant<int, kEpiCC>{});
+    else if (a.diff2_init)
+        epilogue(std::integral_constant<int, kEpiInit>{});
+    else
+        epilogue(std::integral_constant<int, kEpiPlain>{});
 }
 
 // ---------------------------------------------------------------------------

@@ -221,10 +221,9 @@ def test_batched_c1(c1, lib, torch_cuda, form):
         assert got[bidx[i]] == got.min()
 
 
-def test_batched_ragged_multiclass(lib, torch_cuda):
+def _ragged_problem():
     """Two classes, uneven translation counts (3..37 shifts), 5 particles."""
     from relion_amd import synth
-    torch = torch_cuda
     vols = [synth.fourier_volume(synth.blob_map(32, seed=s)) for s in (3, 4)]
     g = synth.CoarseGeometry(imgY=16, maxR=8)
     eul = synth.orientation_set(0, 8)
@@ -235,8 +234,15 @@ def test_batched_ragged_multiclass(lib, torch_cuda):
         txs.append(tx)
         tys.append(ty)
         offs.append(offs[-1] + tx.size)
-    prob = synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
+    return synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
                                np.concatenate(txs), np.concatenate(tys), pb, 32)
+
+
+def test_batched_ragged_multiclass(lib, torch_cuda):
+    """Two classes, uneven translation counts (3..37 shifts), 5 particles."""
+    torch = torch_cuda
+    prob = _ragged_problem()
+    vols = prob.vols
     off = lib.output_offsets(prob.trans_offset, 2, prob.orientation_num)
     refs = [np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(2)])
             for i in range(prob.n_particles)]
@@ -255,6 +261,46 @@ def test_batched_ragged_multiclass(lib, torch_cuda):
                     _assert_close(d[off[i]:off[i + 1]], refs[i].ravel(), form,
                                   f"ragged variant {v} ({tm}x{tn}) particle {i}")
                     assert bidx[i] == oracle.argmin_flat(refs[i]), (form, v, i)
+                plan.close()
+    finally:
+        lib.set_variant_override(-1)
+
+
+def test_batched_diff2_init_all_variants(lib, torch_cuda):
+    """d_diff2_init is added to every score in float64 before the one rounding
+    to float32, and the per-particle minimum is taken over the sums: for
+    every kernel and tile shape on the ragged two-class batch, against the
+    same plan run without it (sum within one rounding of each, minimum and
+    its index on the sums)."""
+    torch = torch_cuda
+    prob = _ragged_problem()
+    off = lib.output_offsets(prob.trans_offset, 2, prob.orientation_num)
+    projs = [lib.Projector(v) for v in prob.vols]
+    rng = np.random.default_rng(23)
+    try:
+        for form in ("fp32", "bf16x3", "fp16"):
+            for v, tm, tn in lib.variants(form):
+                lib.set_variant_override(v)
+                plan = lib.plan_for_problem(prob, projs, form)
+                t = lib.problem_tensors(prob)
+                plan.execute(t)
+                torch.cuda.synchronize()
+                d0 = t["diff2"].cpu().numpy().astype(np.float64)
+                init = (rng.standard_normal(d0.size) * d0.std()).astype(np.float32)
+                t["diff2_init"] = torch.as_tensor(init).cuda()
+                t["diff2"].fill_(float("nan"))
+                plan.execute(t)
+                torch.cuda.synchronize()
+                d1 = t["diff2"].cpu().numpy().astype(np.float64)
+                err = np.abs(d1 - (d0 + init))
+                bound = 1.5e-7 * (np.abs(d0) + np.abs(d1)) + 1e-30
+                assert np.all(err <= bound), (form, v, float((err / bound).max()))
+                best = t["best_diff2"].cpu().numpy()
+                bidx = t["best_index"].cpu().numpy()
+                for i in range(prob.n_particles):
+                    got = d1[off[i]:off[i + 1]]
+                    assert got[bidx[i]] == got.min(), (form, v, i)
+                    assert abs(best[i] - got[bidx[i]]) <= 1e-6 * abs(best[i]), (form, v, i)
                 plan.close()
     finally:
         lib.set_variant_override(-1)
