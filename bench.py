@@ -49,6 +49,7 @@ def time_form(lib, torch, prob, projs, form, steps, warmup, tune=True):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     cross_ms = plan.timing_read(steps)
+    log(f"{form}: cross term per timed step (ms): " + " ".join(f"{m:.3f}" for m in cross_ms))
     res = dict(form=form, ms_per_step=(t1 - t0) * 1e3 / steps,
                cross_ms=sum(cross_ms) / len(cross_ms), variant=plan.tile(),
                plan_bytes=plan.device_bytes(), autotuned=tuned is not None,
@@ -250,7 +251,10 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    # The first ~10 steps of a freshly created plan run up to 8 % slower than
+    # the steady state (the step-time log shows them settling); the default
+    # warm-up covers them.
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--forms", default="bf16x3,fp32")
