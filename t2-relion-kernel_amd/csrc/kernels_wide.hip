@@ -719,6 +719,15 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             // Orientation-major, so that the group's fragment rows of one
             // orientation -- adjacent runs of one output row -- are stored
             // back to back.
+            // output offset of (orientation o0 + rb0 + r16, the lane's first
+            // row); column slice fb adds 16 fb orientations of T translations
+            size_t k00[EG];
+            static_for<0, EG>([&](auto e_c) {
+                constexpr int e = decltype(e_c)::value;
+                const int T = rT[e][0];
+                k00[e] = (size_t)C * O * e_b0[e] + (size_t)cls * O * T +
+                         (size_t)(o0 + rb0 + r16) * T + rt[e][0];
+            });
 #pragma unroll
             for (int fb = 0; fb < FN; ++fb) {
                 const int o = o0 + rb0 + fb * 16 + r16;
@@ -726,9 +735,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                     constexpr int e = decltype(e_c)::value;
                     constexpr int fa = g * EG + e;
                     if (o < O && e_same[e]) {
-                        const int T = rT[e][0];
-                        const size_t k0 = (size_t)C * O * e_b0[e] + (size_t)cls * O * T +
-                                          (size_t)o * T + rt[e][0];
+                        const size_t k0 = k00[e] + (size_t)(fb * 16 * rT[e][0]);
                         float init[4] = {0.f, 0.f, 0.f, 0.f};
                         if constexpr (MODE == kEpiInit) {
                             const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
