@@ -95,9 +95,22 @@ int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ
  * accepted and unused, as in the reference's 2D-data path.  do_CC != 0
  * writes the cross-correlation score -sum corr Re(P conj Xs) /
  * sqrt(sum corr |P|^2) instead (diff2.cuh:691-809, overwriting, with that
- * kernel's pixel-coordinate rule).  data_is_3D != 0 -> RC_ERR_UNSUPPORTED.  Needs scratch device memory; it is
- * taken from an internal per-device cache that grows on demand (the only
- * entry point that may allocate). */
+ * kernel's pixel-coordinate rule).  data_is_3D != 0 -> RC_ERR_UNSUPPORTED.
+ *
+ * Work split: the pixel axis is cut into slices (rc_coarse_split) so that
+ * one image fills the device; each slice writes float32 partial sums and a
+ * second kernel adds the slices in a fixed order in float64.  The slice count
+ * depends on (translation_num, orientation_num, image_size) only, so a
+ * shape always gives bit-identical results.
+ *
+ * Scratch: the reference signature has no workspace, so the call keeps one
+ * scratch buffer per (device, stream) -- calls on different streams (e.g.
+ * RELION's host threads, each with its own stream) never share one, and
+ * calls on one stream are ordered by it.  The first call on a stream, and a
+ * call needing more than that stream has used before, allocates (and frees
+ * the smaller buffer after synchronising that stream): such calls must not
+ * be captured into a hipGraph.  All other calls neither allocate nor
+ * synchronise.  rc_scratch_release frees a stream's buffer. */
 int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
                                const float *d_trans_x, const float *d_trans_y,
                                const float *d_trans_z, const float *d_corr_img,
@@ -107,10 +120,24 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
                                unsigned long image_size, void *stream, int do_CC,
                                int data_is_3D);
 
+/* Frees the scratch rc_run_diff2_kernel_coarse keeps for `stream` on the
+ * current device (synchronises that stream first).  No counterpart in the
+ * reference, whose kernels keep no state. */
+int rc_scratch_release(void *stream);
+
 /* Replaces CoarseMatrixKernel*::get_workspace_size_bytes
- * (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:69-88). */
+ * (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:69-88): image staging plus the
+ * split-K partial sums (ksplit * orientation_num * (translation_num + 1)
+ * floats when rc_coarse_split gives ksplit > 1). */
 size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int image_size,
                                  int full_image_size, int form);
+
+/* Slices of the pixel axis the single-image calls use for a shape (the
+ * counterpart of the split-K count of coarse_scheduler.cuh:211-213, here
+ * min(chunks, ceil(512 / tiles)) bounded by 32 MiB of partials, with no empty
+ * slice): ksplit slices of kc_per_split 32-pixel chunks.  Diagnostic. */
+int rc_coarse_split(int translation_num, int orientation_num, int image_size, int form,
+                    int *ksplit, int *kc_per_split);
 
 /* Replaces CoarseMatrixKernel*::run (coarse_splitImg_BCF_PMO_TMDB_NFUSE.cuh:90-131).
  * d_real/d_imag/d_corr: the first image_size pixels are inside the
@@ -127,6 +154,33 @@ int rc_coarse_run(int translation_num, int orientation_num, int image_size, int 
                   const rc_projector_kernel *projector, const float *d_corr,
                   const float *d_diff2s, float *d_diff2s_dest, void *stream,
                   const float *d_coor_xy, void *d_workspace, size_t workspace_bytes);
+
+/* ---------------------------------------------------------------------- */
+/* In-radius partition (FilterCoarseParamImage, main.cu:88-151)            */
+/* ---------------------------------------------------------------------- */
+/* Pixel i of an unfiltered half-plane image has x = i % imgX, y = i / imgX,
+ * y -= imgY if y > maxR (diff2.cuh:182-188), and is inside when the float
+ * (x*x + y*y) * pad * pad <= maxR2_padded (the view's geometry).  The
+ * filtered order keeps inside pixels first in index order and stores the
+ * outside pixels from the last position down.
+ *
+ * Host form: host_order[j] = source pixel of filtered position j (or NULL),
+ * *n_inside = number of inside pixels (the `npix` of a batch).  Only the
+ * view's geometry is read (its projector may be NULL). */
+int rc_filter_coarse_order(const rc_projector_kernel *view, int image_size, int *host_order,
+                           int *n_inside);
+
+/* Device form for a batch, stream-ordered, no allocation: image b reads
+ * d_fimg[b][2*image_size] (real block, then imaginary block, as the
+ * reference's fimg) and d_corr[b][image_size]; it writes d_real, d_imag,
+ * d_corr_out [b][image_size] in the filtered order, d_coor_xy[2*image_size]
+ * (x, y) of each filtered position once (or NULL), and d_filtered_size[b]
+ * (or NULL).  The outputs are the d_real / d_imag / d_corr / d_coor_xy of an
+ * rc_batch_desc with full_npix = image_size and npix = *n_inside. */
+int rc_filter_coarse_images(const rc_projector_kernel *view, int image_size, int n_images,
+                            const float *d_fimg, const float *d_corr, float *d_real,
+                            float *d_imag, float *d_corr_out, float *d_coor_xy,
+                            int *d_filtered_size, void *stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batched entry points (no reference counterpart: the reference calls the */
@@ -191,19 +245,20 @@ int rc_plan_timing_read(rc_plan *plan, float *ms, int n, int *n_written);
 int rc_plan_capture(rc_plan *plan, const rc_batch_desc *desc, void *stream);
 int rc_plan_replay(rc_plan *plan, void *stream);
 
-/* Autotuner: time every cross-term variant of desc's form on desc's shape
- * (one plan per variant in turn, two warm-up and `iters` timed launches,
- * HIP events on `stream`; two sweeps over the table, each variant's better
- * median compared), keep the fastest in a
- * process-wide table keyed
- * by the shape (particles, classes, orientations, pixels, form, row groups)
- * that later rc_plan_create calls consult, and return it.  Allocates and
- * synchronises the stream; not for graph capture. */
+/* Autotuner: time the cross-term variants of desc's form on desc's shape
+ * (one plan per variant in turn, HIP events on `stream`): a screen of one
+ * timed launch per variant, then two sweeps of two untimed and `iters`
+ * timed launches over the variants within 25 % of the fastest screen, each
+ * keeping its better median.  The fastest is kept in a process-wide table
+ * keyed by the shape (particles, classes, orientations, pixels, form, row
+ * groups) that later rc_plan_create calls consult, and returned.
+ * Allocates and synchronises the stream; not for graph capture. */
 int rc_autotune(const rc_batch_desc *desc, void *stream, int iters, int *best_variant,
                 float *best_ms);
 
 /* Force the cross-term kernel variant of later rc_plan_create calls
- * (-1 = automatic).  Used by the autotuner. */
+ * (-1 = automatic).  Process-wide; the autotuner does not use it (its
+ * sweep names each variant to its own plans). */
 int rc_set_variant_override(int variant);
 
 /* Number of cross-term kernel variants and their tile shapes. */

@@ -101,6 +101,22 @@ void oracle_project_f64(const float *vre, const float *vim, int mdlX, int mdlY, 
         project_f64(&v, (double)pad, maxR2_padded, e9, xs[p], ys[p], out_re + p, out_im + p);
 }
 
+/* Projections of O orientations at n pixels (float64, OpenMP over the
+ * orientations): out[o*n + p], the per-orientation loop of oracle_diff2_f64
+ * without the score.  Used by the expanded-form evaluator in oracle.py. */
+void oracle_project_many_f64(const float *vre, const float *vim, int mdlX, int mdlY, int mdlZ,
+                             int initY, int initZ, float pad, int maxR2_padded,
+                             const float *eulers, int O, int n, const double *xs,
+                             const double *ys, double *out_re, double *out_im)
+{
+    vol_t v = {vre, vim, mdlX, mdlY, mdlZ, initY, initZ};
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int o = 0; o < O; ++o)
+        for (int p = 0; p < n; ++p)
+            project_f64(&v, (double)pad, maxR2_padded, eulers + 9 * (size_t)o, xs[p], ys[p],
+                        out_re + (size_t)o * n + p, out_im + (size_t)o * n + p);
+}
+
 /* Phase-shift one image by one translation (float64). */
 void oracle_translate_f64(int n, const double *xs, const double *ys, float tx, float ty,
                           const float *re, const float *im, double *out_re, double *out_im)

@@ -38,6 +38,10 @@ def lib():
                                          C.c_float, C.c_int, _f32p, C.c_int, _f64p, _f64p,
                                          _f64p, _f64p]
         L.oracle_project_f64.restype = None
+        L.oracle_project_many_f64.argtypes = [_f32p, _f32p, C.c_int, C.c_int, C.c_int, C.c_int,
+                                               C.c_int, C.c_float, C.c_int, _f32p, C.c_int,
+                                               C.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.oracle_project_many_f64.restype = None
         L.oracle_translate_f64.argtypes = [C.c_int, _f64p, _f64p, C.c_float, C.c_float,
                                            _f32p, _f32p, _f64p, _f64p]
         L.oracle_translate_f64.restype = None
@@ -163,6 +167,59 @@ def problem_diff2(prob, i, cls=0, orient=None, init=None):
     eul = prob.eulers if orient is None else prob.eulers[orient]
     return diff2_f64(prob.vols[cls], prob.geom.maxR2_padded, eul, tx, ty, pb.real[i], pb.imag[i],
                      pb.corr[i], pb.coor_xy[:, 0], pb.coor_xy[:, 1], n_proj=pb.npix, init=init)
+
+
+def projections(v, maxR2_padded, eulers, xs, ys):
+    """float64 projections ``[O, n]`` (complex128) of every orientation at
+    pixels (xs, ys): the projection step of ``diff2_f64``."""
+    eulers = _c32(eulers).reshape(-1, 9)
+    O = eulers.shape[0]
+    xs, ys = _c64(xs), _c64(ys)
+    re = np.zeros(O * xs.size)
+    im = np.zeros(O * xs.size)
+    lib().oracle_project_many_f64(*_vol_args(v), int(maxR2_padded), eulers.ravel(), O, xs.size,
+                                  xs, ys, re, im)
+    return (re + 1j * im).reshape(O, xs.size)
+
+
+def diff2_expanded(P, trans_x, trans_y, re, im, corr, xs, ys, n_proj):
+    """float64 coarse diff2 ``[O, T]`` of one image in the expanded form
+    sum w/2 |P|^2 + sum w/2 |X|^2 - sum w Re(P conj(X e^{i phi})), with the
+    projections ``P [O, n_proj]`` from ``projections`` and the pixel sums
+    as float64 matrix products.  The same quantity as ``diff2_f64`` (the
+    direct form sum w/2 |P - Xs|^2), which pins it in the CPU tests; it is
+    here because it costs one complex GEMM per image instead of a sincos
+    per (orientation, translation, pixel)."""
+    xs, ys = _c64(xs), _c64(ys)
+    w = np.asarray(corr, np.float64)
+    X = np.asarray(re, np.float64) + 1j * np.asarray(im, np.float64)
+    tx = np.asarray(trans_x, np.float32).astype(np.float64)
+    ty = np.asarray(trans_y, np.float32).astype(np.float64)
+    xn = 0.5 * float(np.sum(w * (X.real ** 2 + X.imag ** 2)))
+    k = int(n_proj)
+    phi = tx[:, None] * xs[None, :k] + ty[:, None] * ys[None, :k]
+    A = (w[:k] * X[:k])[None, :] * np.exp(1j * phi)          # [T, k]: w Xs
+    pn = (P.real ** 2 + P.imag ** 2) @ (0.5 * w[:k])          # [O]
+    cross = (P @ np.conj(A).T).real                           # [O, T]
+    return pn[:, None] + xn - cross
+
+
+def problem_diff2_expanded(prob, particles, cls=0, P=None):
+    """``diff2_expanded`` of the given particles of a synthetic problem
+    against class ``cls``: {i: [O, T_i]}.  ``P`` may hold the class's
+    projections from an earlier call (``projections`` at the in-radius
+    pixels)."""
+    pb = prob.particles
+    xs, ys = pb.coor_xy[:, 0], pb.coor_xy[:, 1]
+    if P is None:
+        P = projections(prob.vols[cls], prob.geom.maxR2_padded, prob.eulers, xs[:pb.npix],
+                        ys[:pb.npix])
+    out = {}
+    for i in particles:
+        tx, ty = prob.trans_of(i)
+        out[int(i)] = diff2_expanded(P, tx, ty, pb.real[i], pb.imag[i], pb.corr[i], xs, ys,
+                                     pb.npix)
+    return out
 
 
 def error_stats(a, ref):

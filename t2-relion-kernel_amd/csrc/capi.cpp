@@ -4,12 +4,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
 #include <tuple>
 #include <mutex>
 #include <string>
@@ -79,16 +81,64 @@ static int check_kernel_view(const rc_projector_kernel *k)
 }
 
 // ---------------------------------------------------------------------------
-// single-image workspace
+// single-image workspace and split-K
 // ---------------------------------------------------------------------------
+// One image alone gives few tiles (C1: 3, a 113-translation particle at 4608
+// orientations: 36), far fewer than the 256 CUs.  The fused kernel then
+// splits the pixel axis into slices that each write float32 partial sums
+// (cross term per (orientation, translation), |P|^2 norm per orientation)
+// to the workspace, and a second kernel adds the slices in a fixed order in
+// float64.  The slice count follows from the shape alone -- never from the
+// device or from timing -- so a shape always gives the same bits.
+constexpr int kSplitTargetBlocks = 512;            // workgroups the split aims for
+constexpr size_t kSplitMaxBytes = 32ull << 20;     // cap of the cross-term partials
+
+struct SingleShape {
+    int variant;             // fused cross-kernel variant
+    int tiles;               // row tiles x column tiles
+    int nkc;                 // 32-pixel chunks
+    int ksplit, kc_per_split;
+};
+
+static int pick_fused_variant(int form, int T)
+{
+    (void)form;
+    if (T <= 32) return 2;   // 32 x 256
+    if (T <= 64) return 1;   // 64 x 128
+    return 0;                // 128 x 128
+}
+
+static SingleShape single_shape(int T, int O, int npix, int form)
+{
+    SingleShape sh{};
+    sh.nkc = (int)(align_up((size_t)std::max(npix, 1), kChunk) / kChunk);
+    sh.ksplit = 1;
+    sh.kc_per_split = sh.nkc;
+    if (form == RC_FORM_FP32_DIRECT || T <= 0 || O <= 0) return sh;
+    sh.variant = pick_fused_variant(form, T);
+    const VariantInfo *info = variant_info(form, true, sh.variant);
+    const long long rows = (long long)((T + 15) / 16) * 16;
+    const long long tiles = ((rows + info->tile_m - 1) / info->tile_m) *
+                            (((long long)O + info->tile_n - 1) / info->tile_n);
+    sh.tiles = (int)std::min<long long>(tiles, 1 << 30);
+    long long want = (kSplitTargetBlocks + tiles - 1) / tiles;
+    const long long by_bytes = (long long)(kSplitMaxBytes / ((size_t)T * O * sizeof(float)));
+    want = std::max(1ll, std::min({want, (long long)sh.nkc, by_bytes}));
+    sh.kc_per_split = (int)((sh.nkc + want - 1) / want);
+    sh.ksplit = (sh.nkc + sh.kc_per_split - 1) / sh.kc_per_split;   // no empty slice
+    return sh;
+}
+
 struct SingleWs {
     float2 *xw, *coords;
     double *xnorm;
     float *half_corr, *ascale;
+    float *part_c, *part_pn;     // split-K partials (NULL without a split)
     size_t bytes;
 };
 
-static SingleWs single_layout(unsigned char *base, int npix)
+static SingleWs single_layout(unsigned char *base, int npix, int T, int O,
+                              const SingleShape &sh)
 {
     const int kpad = (int)align_up((size_t)std::max(npix, 1), kChunk);
     Carve c{base};
@@ -98,16 +148,13 @@ static SingleWs single_layout(unsigned char *base, int npix)
     w.xnorm = c.take<double>(1);
     w.half_corr = c.take<float>(kpad);
     w.ascale = c.take<float>(1);
+    w.part_c = w.part_pn = nullptr;
+    if (sh.ksplit > 1) {
+        w.part_c = c.take<float>((size_t)sh.ksplit * O * T);
+        w.part_pn = c.take<float>((size_t)sh.ksplit * O);
+    }
     w.bytes = align_up(c.off, 256);
     return w;
-}
-
-static int pick_fused_variant(int form, int T)
-{
-    (void)form;
-    if (T <= 32) return 2;   // 32 x 256
-    if (T <= 64) return 1;   // 64 x 128
-    return 0;                // 128 x 128
 }
 
 static int run_single(int T, int O, int npix, int full_npix, int form, const float *eulers,
@@ -129,7 +176,8 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                "unknown form %d", form);
     RC_REQUIRE(eulers && trans_x && trans_y && real && imag && corr && dest, RC_ERR_INVALID,
                "NULL device pointer");
-    SingleWs w = single_layout((unsigned char *)ws, npix);
+    const SingleShape sh = single_shape(T, O, npix, form);
+    SingleWs w = single_layout((unsigned char *)ws, npix, T, O, sh);
     RC_REQUIRE(ws && ws_bytes >= w.bytes, RC_ERR_INVALID,
                "workspace of %zu bytes is smaller than the %zu bytes needed", ws_bytes, w.bytes);
     const int kpad = (int)align_up((size_t)npix, kChunk);
@@ -143,7 +191,7 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                              full_npix, init, dest, stream));
         return RC_OK;
     }
-    const int vi = pick_fused_variant(form, T);
+    const int vi = sh.variant;
     const VariantInfo *info = variant_info(form, true, vi);
     CrossArgs a{};
     a.groups = nullptr;
@@ -171,45 +219,138 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
     a.part_min = nullptr;
     a.part_idx = nullptr;
     a.cc = cc ? 1 : 0;
+    a.ksplit = sh.ksplit;
+    a.kc_per_split = sh.kc_per_split;
+    a.part_c = w.part_c;
+    a.part_pn = w.part_pn;
     RC_HIP(launch_cross(form, true, vi, a, stream));
+    if (sh.ksplit > 1)
+        RC_HIP(launch_splitk_assemble(O, T, sh.ksplit, w.part_c, w.part_pn, w.xnorm, init, dest,
+                                      cc ? 1 : 0, stream));
     return RC_OK;
 }
 
-// per-device scratch for the runDiff2KernelCoarse mirror (grows on demand)
-struct ScratchCache {
-    std::mutex mu;
-    std::vector<std::pair<void *, size_t>> per_dev;
-};
-static ScratchCache g_scratch;
+size_t single_workspace_bytes(int T, int O, int npix, int form)
+{
+    if (npix <= 0 || T <= 0 || O <= 0) return 0;
+    return single_layout(nullptr, npix, T, O, single_shape(T, O, npix, form)).bytes;
+}
 
-static int scratch_for(size_t bytes, void **out)
+int single_split(int T, int O, int npix, int form, int *ksplit, int *kc_per_split)
+{
+    const SingleShape sh = single_shape(T, O, npix, form);
+    if (ksplit) *ksplit = sh.ksplit;
+    if (kc_per_split) *kc_per_split = sh.kc_per_split;
+    return RC_OK;
+}
+
+int run_single_ws(int T, int O, int npix, int full_npix, int form, const float *eulers,
+                  const float *trans_x, const float *trans_y, const float *real, const float *imag,
+                  const rc_projector_kernel *pk, const float *corr, const float *init, float *dest,
+                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc)
+{
+    return run_single(T, O, npix, full_npix, form, eulers, trans_x, trans_y, real, imag, pk, corr,
+                      init, dest, stream, coor_xy, ws, ws_bytes, cc);
+}
+
+// Scratch of rc_run_diff2_kernel_coarse, one buffer per (device, stream):
+// calls on different streams never share memory, and calls on one stream are
+// ordered by that stream.  The entry's mutex is held while a call enqueues,
+// so a call that grows the buffer cannot free it under another thread that
+// is still launching with it; the old buffer is freed only after the stream
+// has drained.
+struct ScratchEntry {
+    std::mutex mu;
+    void *ptr = nullptr;
+    size_t bytes = 0;
+};
+static std::mutex g_scratch_mu;
+static std::map<std::pair<int, hipStream_t>, std::unique_ptr<ScratchEntry>> g_scratch;
+
+static int scratch_for(hipStream_t stream, size_t bytes, std::unique_lock<std::mutex> &held,
+                       void **out)
 {
     int dev = 0;
     RC_HIP(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lk(g_scratch.mu);
-    if ((int)g_scratch.per_dev.size() <= dev) g_scratch.per_dev.resize(dev + 1, {nullptr, 0});
-    auto &e = g_scratch.per_dev[dev];
-    if (e.second < bytes) {
-        if (e.first) {
-            RC_HIP(hipDeviceSynchronize());
-            RC_HIP(hipFree(e.first));
-            e = {nullptr, 0};
+    ScratchEntry *e;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto &slot = g_scratch[{dev, stream}];
+        if (!slot) slot.reset(new ScratchEntry);
+        e = slot.get();
+    }
+    held = std::unique_lock<std::mutex>(e->mu);
+    if (e->bytes < bytes) {
+        if (e->ptr) {
+            RC_HIP(hipStreamSynchronize(stream));
+            RC_HIP(hipFree(e->ptr));
+            e->ptr = nullptr;
+            e->bytes = 0;
         }
+        const size_t want = align_up(bytes, 1 << 20);
         void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) {
-            set_error("hipMalloc of %zu scratch bytes failed", bytes);
+        if (hipMalloc(&p, want) != hipSuccess) {
+            set_error("hipMalloc of %zu scratch bytes failed", want);
             return RC_ERR_NO_MEMORY;
         }
-        e = {p, bytes};
+        e->ptr = p;
+        e->bytes = want;
     }
-    *out = e.first;
+    *out = e->ptr;
     return RC_OK;
+}
+
+static int scratch_release(hipStream_t stream)
+{
+    int dev = 0;
+    RC_HIP(hipGetDevice(&dev));
+    std::unique_ptr<ScratchEntry> e;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch.find({dev, stream});
+        if (it == g_scratch.end()) return RC_OK;
+        e = std::move(it->second);
+        g_scratch.erase(it);
+    }
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (e->ptr) {
+        RC_HIP(hipStreamSynchronize(stream));
+        RC_HIP(hipFree(e->ptr));
+    }
+    return RC_OK;
+}
+
+int check_drop_in(const rc_projector_kernel *pk, unsigned long O, unsigned long T,
+                  unsigned long image_size, int data_is_3D)
+{
+    RC_REQUIRE(!data_is_3D, RC_ERR_UNSUPPORTED, "3D data (data_is_3D) is not implemented");
+    if (int st = check_kernel_view(pk)) return st;
+    RC_REQUIRE(O > 0 && T > 0 && image_size > 0 && O < (1ul << 30) &&
+                   T <= (unsigned long)kMaxTrans && image_size <= (unsigned long)kMaxPix,
+               RC_ERR_LIMIT, "sizes out of range");
+    RC_REQUIRE(image_size <= (unsigned long)pk->imgX * pk->imgY, RC_ERR_INVALID,
+               "image_size %lu exceeds imgX*imgY", image_size);
+    return RC_OK;
+}
+
+// Host statement of the in-radius rule of FilterCoarseParamImage
+// (main.cu:88-151): int (x*x + y*y), then float products with the padding
+// factor, compared with maxR2_padded as a float.
+static bool filter_inside_host(int i, const rc_projector_kernel &k, int &x, int &y)
+{
+    x = i % k.imgX;
+    y = i / k.imgX;
+    if (y > k.maxR) y -= k.imgY;
+    const float r2 = (float)(x * x + y * y) * k.padding_factor * k.padding_factor;
+    return r2 <= (float)k.maxR2_padded;
 }
 
 // ---------------------------------------------------------------------------
 // batched plans
 // ---------------------------------------------------------------------------
-static int g_variant_override = -1;
+// Variant forced on every later rc_plan_create (rc_set_variant_override);
+// -1 = automatic.  Atomic: plans may be created from several host threads.
+static std::atomic<int> g_variant_override{-1};
 
 struct TuneKey {
     int P, C, O, npix, form, groups;
@@ -321,7 +462,7 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
 
 namespace rc {
 
-int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
+int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab, int forced_variant)
 {
     if (int st = validate_batch(d, false)) return st;
     p->P = d->n_particles;
@@ -346,7 +487,7 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab)
     tab.row_part.resize(p->n_rows);
     for (int i = 0; i < p->P; ++i)
         for (int r = p->trans_offset[i]; r < p->trans_offset[i + 1]; ++r) tab.row_part[r] = i;
-    int vi = g_variant_override;
+    int vi = forced_variant >= 0 ? forced_variant : g_variant_override.load();
     if (vi < 0) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(d));
@@ -671,32 +812,27 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
 {
     (void)d_trans_z;
     (void)local_sqrtXi2;
-    RC_REQUIRE(!data_is_3D, RC_ERR_UNSUPPORTED, "3D data (data_is_3D) is not implemented");
-    if (int st = check_kernel_view(projector)) return st;
-    RC_REQUIRE(orientation_num > 0 && translation_num > 0 && image_size > 0 &&
-                   orientation_num < (1ul << 30) && translation_num <= (unsigned long)kMaxTrans &&
-                   image_size <= (unsigned long)kMaxPix,
-               RC_ERR_LIMIT, "sizes out of range");
-    RC_REQUIRE(image_size <= (unsigned long)projector->imgX * projector->imgY, RC_ERR_INVALID,
-               "image_size %lu exceeds imgX*imgY", image_size);
-    const SingleWs w = single_layout(nullptr, (int)image_size);
+    if (int st = check_drop_in(projector, orientation_num, translation_num, image_size,
+                               data_is_3D))
+        return st;
+    const size_t need = single_workspace_bytes((int)translation_num, (int)orientation_num,
+                                               (int)image_size, RC_FORM_FP32);
+    std::unique_lock<std::mutex> held;
     void *ws = nullptr;
-    if (int st = scratch_for(w.bytes, &ws)) return st;
+    if (int st = scratch_for((hipStream_t)stream, need, held, &ws)) return st;
     return run_single((int)translation_num, (int)orientation_num, (int)image_size,
                       (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real,
                       d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
-                      (hipStream_t)stream, nullptr, ws, w.bytes, do_CC != 0);
+                      (hipStream_t)stream, nullptr, ws, need, do_CC != 0);
 }
+
+int rc_scratch_release(void *stream) { return scratch_release((hipStream_t)stream); }
 
 size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int image_size,
                                  int full_image_size, int form)
 {
-    (void)translation_num;
-    (void)orientation_num;
     (void)full_image_size;
-    (void)form;
-    if (image_size <= 0) return 0;
-    return single_layout(nullptr, image_size).bytes;
+    return single_workspace_bytes(translation_num, orientation_num, image_size, form);
 }
 
 int rc_coarse_run(int translation_num, int orientation_num, int image_size, int full_image_size,
@@ -714,53 +850,69 @@ int rc_coarse_run(int translation_num, int orientation_num, int image_size, int 
                       workspace_bytes);
 }
 
+// Times `timed` launches of variant v on desc (after `warm` untimed ones)
+// with HIP events on the stream; returns their median in *ms.
+static int time_variant(const rc_batch_desc *d, hipStream_t stream, int v, int warm, int timed,
+                        float *ms)
+{
+    rc_plan *p = nullptr;
+    int st = plan_create_internal(d, v, &p);
+    if (st == RC_OK) st = rc_plan_timing(p, timed);
+    for (int k = 0; st == RC_OK && k < warm; ++k) st = rc_plan_execute(p, d, stream);
+    if (st == RC_OK && hipStreamSynchronize(stream) != hipSuccess) {
+        set_error("stream synchronisation failed while tuning");
+        st = RC_ERR_DEVICE;
+    }
+    if (st == RC_OK) p->launches = 0;
+    for (int k = 0; st == RC_OK && k < timed; ++k) st = rc_plan_execute(p, d, stream);
+    std::vector<float> t(timed);
+    int got = 0;
+    if (st == RC_OK) st = rc_plan_timing_read(p, t.data(), timed, &got);
+    if (p) rc_plan_destroy(p);
+    if (st != RC_OK) return st;
+    // median: robust to the odd slow launch (clock ramps, other streams)
+    std::sort(t.begin(), t.begin() + got);
+    *ms = got == 0 ? 0.f : (got & 1) ? t[got / 2] : 0.5f * (t[got / 2 - 1] + t[got / 2]);
+    return RC_OK;
+}
+
 int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_variant,
                 float *best_ms)
 {
     if (int st = validate_batch(d, true)) return st;
     RC_REQUIRE(iters > 0 && iters <= 1000, RC_ERR_INVALID, "iters must be 1..1000");
+    const hipStream_t s = (hipStream_t)stream;
     const int n = variant_count(d->form, false);
-    const int saved = g_variant_override;
+    // Each plan is made with its variant named explicitly, so plans created
+    // meanwhile by other threads are not affected by the sweep.
+    // Screen: one untimed and one timed launch per variant.
+    std::vector<float> first(n);
+    float fastest = 0.f;
+    for (int v = 0; v < n; ++v) {
+        if (int st = time_variant(d, s, v, 1, 1, &first[v])) return st;
+        if (v == 0 || first[v] < fastest) fastest = first[v];
+    }
+    // Refine the variants within 25 % of the fastest screen: two sweeps of
+    // two untimed and `iters` timed launches, each keeping its better
+    // median (a variant timed early is not penalised by a clock still
+    // ramping up).
+    std::vector<int> cand;
+    for (int v = 0; v < n; ++v)
+        if (first[v] <= 1.25f * fastest) cand.push_back(v);
+    std::vector<float> vt(n, -1.f);
+    for (int sweep = 0; sweep < 2; ++sweep)
+        for (int v : cand) {
+            float t = 0.f;
+            if (int st = time_variant(d, s, v, 2, iters, &t)) return st;
+            if (vt[v] < 0.f || t < vt[v]) vt[v] = t;
+        }
     int best = -1;
     float best_t = 0.f;
-    std::vector<float> ms(iters);
-    // two sweeps over the table, each variant keeping its better median: a
-    // variant timed early is not penalised by the clock still ramping up
-    std::vector<float> vt(n, -1.f);
-    for (int sweep = 0; sweep < 2 * n; ++sweep) {
-        const int v = sweep % n;
-        g_variant_override = v;
-        rc_plan *p = nullptr;
-        int st = rc_plan_create(d, &p);
-        if (st == RC_OK) st = rc_plan_timing(p, iters);
-        // two untimed launches first
-        for (int k = 0; st == RC_OK && k < 2; ++k) st = rc_plan_execute(p, d, stream);
-        if (st == RC_OK && hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
-            set_error("stream synchronisation failed while tuning");
-            st = RC_ERR_DEVICE;
-        }
-        if (st == RC_OK) p->launches = 0;
-        for (int k = 0; st == RC_OK && k < iters; ++k) st = rc_plan_execute(p, d, stream);
-        int got = 0;
-        if (st == RC_OK) st = rc_plan_timing_read(p, ms.data(), iters, &got);
-        if (p) rc_plan_destroy(p);
-        if (st != RC_OK) {
-            g_variant_override = saved;
-            return st;
-        }
-        // median: robust to the odd slow launch (clock ramps, other streams)
-        std::sort(ms.begin(), ms.begin() + got);
-        const float t = got == 0 ? 0.f
-                        : (got & 1) ? ms[got / 2]
-                                    : 0.5f * (ms[got / 2 - 1] + ms[got / 2]);
-        if (vt[v] < 0.f || t < vt[v]) vt[v] = t;
-    }
-    for (int v = 0; v < n; ++v)
+    for (int v : cand)
         if (best < 0 || vt[v] < best_t) {
             best = v;
             best_t = vt[v];
         }
-    g_variant_override = saved;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tuned[tune_key(d)] = best;
@@ -772,7 +924,8 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
 
 int rc_set_variant_override(int variant)
 {
-    g_variant_override = variant;
+    RC_REQUIRE(variant >= -1, RC_ERR_INVALID, "variant override %d (must be >= -1)", variant);
+    g_variant_override.store(variant);
     return RC_OK;
 }
 
@@ -892,13 +1045,19 @@ const char *rc_variant_name(int form, int variant)
     return buf;
 }
 
-int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
+int rc_plan_create(const rc_batch_desc *d, rc_plan **out) { return plan_create_internal(d, -1, out); }
+
+}  // extern "C"
+
+namespace rc {
+
+int plan_create_internal(const rc_batch_desc *d, int forced_variant, rc_plan **out)
 {
     RC_REQUIRE(out, RC_ERR_INVALID, "out is NULL");
     *out = nullptr;
     rc_plan *p = new rc_plan{};
     PlanTables tab;
-    if (int st = plan_setup(p, d, tab)) {
+    if (int st = plan_setup(p, d, tab, forced_variant)) {
         delete p;
         return st;
     }
@@ -928,6 +1087,10 @@ int rc_plan_create(const rc_batch_desc *d, rc_plan **out)
     *out = p;
     return RC_OK;
 }
+
+}  // namespace rc
+
+extern "C" {
 
 int rc_plan_execute(rc_plan *p, const rc_batch_desc *d, void *stream)
 {
@@ -1025,6 +1188,54 @@ int rc_plan_tile(const rc_plan *p, int *variant, int *tile_m, int *tile_n)
     if (variant) *variant = p->variant;
     if (tile_m) *tile_m = p->info.tile_m;
     if (tile_n) *tile_n = p->info.tile_n;
+    return RC_OK;
+}
+
+int rc_coarse_split(int translation_num, int orientation_num, int image_size, int form,
+                    int *ksplit, int *kc_per_split)
+{
+    RC_REQUIRE(translation_num > 0 && orientation_num > 0 && image_size > 0, RC_ERR_INVALID,
+               "sizes must be positive");
+    RC_REQUIRE(form >= RC_FORM_FP32 && form <= RC_FORM_FP32_DIRECT, RC_ERR_INVALID,
+               "unknown form %d", form);
+    return single_split(translation_num, orientation_num, image_size, form, ksplit, kc_per_split);
+}
+
+int rc_filter_coarse_order(const rc_projector_kernel *k, int image_size, int *host_order,
+                           int *n_inside)
+{
+    RC_REQUIRE(k && n_inside, RC_ERR_INVALID, "NULL argument");
+    RC_REQUIRE(k->imgX > 0 && k->imgY > 0 && image_size > 0 &&
+                   (long long)image_size <= (long long)k->imgX * k->imgY,
+               RC_ERR_INVALID, "image_size %d does not fit %d x %d", image_size, k->imgX,
+               k->imgY);
+    int head = 0, tail = image_size - 1;
+    for (int i = 0; i < image_size; ++i) {
+        int x, y;
+        const bool in = filter_inside_host(i, *k, x, y);
+        const int d = in ? head++ : tail--;
+        if (host_order) host_order[d] = i;
+    }
+    *n_inside = head;
+    return RC_OK;
+}
+
+int rc_filter_coarse_images(const rc_projector_kernel *k, int image_size, int n_images,
+                            const float *d_fimg, const float *d_corr, float *d_real,
+                            float *d_imag, float *d_corr_out, float *d_coor_xy,
+                            int *d_filtered_size, void *stream)
+{
+    RC_REQUIRE(k, RC_ERR_INVALID, "NULL projector kernel view");
+    RC_REQUIRE(k->imgX > 0 && k->imgY > 0 && image_size > 0 &&
+                   (long long)image_size <= (long long)k->imgX * k->imgY,
+               RC_ERR_INVALID, "image_size %d does not fit %d x %d", image_size, k->imgX,
+               k->imgY);
+    RC_REQUIRE(n_images > 0 && n_images < (1 << 30), RC_ERR_INVALID, "n_images %d", n_images);
+    RC_REQUIRE(d_fimg && d_corr && d_real && d_imag && d_corr_out, RC_ERR_INVALID,
+               "NULL device pointer");
+    RC_HIP(launch_filter(image_size, n_images, k->imgX, k->imgY, k->maxR, k->padding_factor,
+                         k->maxR2_padded, d_fimg, d_corr, d_real, d_imag, d_corr_out, d_coor_xy,
+                         d_filtered_size, (hipStream_t)stream));
     return RC_OK;
 }
 

@@ -176,6 +176,9 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
 // column reads); the next chunk's global loads are issued before the
 // current chunk's MFMAs.  A[i][k] = half_corr, B[k][n] = |P|^2.  kpad is a
 // multiple of 32; a last half chunk is zero-filled.
+// Summation: float32 products summed in float32 within a 64-pixel chunk
+// only; the chunk sums are added in float64 (a cascaded sum), so the
+// rounding error grows with the chunk length (64), not with the pixel count.
 constexpr int kPnK = 64;
 constexpr int kPnRow = kPnK + 4;
 
@@ -216,13 +219,20 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
             if (b_ok && in) rb[u] = *reinterpret_cast<const float4 *>(bp + k0 + 4 * u);
         }
     };
-    floatx4 acc[2][2];
+    double dacc[2][2][4];
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int y = 0; y < 2; ++y) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dacc[x][y][v] = 0.0;
     fetch(0);
     for (int k0 = 0; k0 < kpad; k0 += kPnK) {
+        floatx4 acc[2][2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -241,6 +251,12 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
             acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
             acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
         }
+#pragma unroll
+        for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dacc[x][y][v] += (double)acc[x][y][v];
     }
     // D[row = 4q + v][col = r16] of each 16 x 16 fragment
 #pragma unroll
@@ -250,7 +266,7 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int i = i0 + wr + 16 * x + 4 * q + v, n = n0 + wc + 16 * y + r16;
-                if (i < P && n < N) pnorm[(size_t)i * N + n] = acc[x][y][v];
+                if (i < P && n < N) pnorm[(size_t)i * N + n] = (float)dacc[x][y][v];
             }
 }
 
@@ -261,6 +277,49 @@ hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half
     dim3 grid(((total + 7) / 8) * 8);
     hipLaunchKernelGGL(pnorm_mfma_kernel, grid, dim3(256), 0, s, n_particles, n_cols, kpad,
                        half_corr, pabs2, pnorm);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// split-K assembly of one image (single-image entry points)
+// ---------------------------------------------------------------------------
+// out[o*T+t] = init + pn + xn - c   (or the CC score -c / sqrt(2 pn)), where
+// c = sum_s part_c[s][o][t] and pn = sum_s part_pn[s][o] are summed in float64
+// over the slices in order s = 0, 1, ..., so the result depends on the slice
+// count only, never on the order the slices finished in.
+__global__ __launch_bounds__(256) void splitk_assemble_kernel(int O, int T, int S,
+                                                              const float *__restrict__ part_c,
+                                                              const float *__restrict__ part_pn,
+                                                              const double *__restrict__ xnorm,
+                                                              const float *init, float *out,
+                                                              int cc)
+{
+    const size_t n = (size_t)O * T;
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int o = (int)(k / T);
+    double c = 0.0, pn = 0.0;
+    for (int sl = 0; sl < S; ++sl) {
+        c += (double)part_c[(size_t)sl * n + k];
+        pn += (double)part_pn[(size_t)sl * O + o];
+    }
+    double d;
+    if (cc) {
+        d = -c / sqrt(2.0 * pn);
+    } else {
+        d = pn + xnorm[0] - c;
+        if (init) d += (double)init[k];
+    }
+    out[k] = (float)d;
+}
+
+hipError_t launch_splitk_assemble(int O, int T, int S, const float *part_c, const float *part_pn,
+                                  const double *xnorm, const float *init, float *out, int cc,
+                                  hipStream_t s)
+{
+    const size_t n = (size_t)O * T;
+    hipLaunchKernelGGL(splitk_assemble_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                       O, T, S, part_c, part_pn, xnorm, init, out, cc);
     return hipGetLastError();
 }
 

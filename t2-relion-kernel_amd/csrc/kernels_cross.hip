@@ -92,6 +92,14 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
     const int r16 = lane & 15;
     const int q = lane >> 4;
     const int nkc = a.kpad / kChunk;
+    // split-K (fused kernels): slice blockIdx.y covers chunks [kc0, kc1)
+    int kc0 = 0, kc1 = nkc;
+    if constexpr (FUSED) {
+        if (a.ksplit > 1) {
+            kc0 = (int)blockIdx.y * a.kc_per_split;
+            kc1 = min(nkc, kc0 + a.kc_per_split);
+        }
+    }
 
     // ---- rows owned by this lane (one per fragment)
     int g_part[FM], g_t0[FM], g_T[FM], g_base[FM];
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
         }
     }
 
-    for (int kc = 0; kc < nkc; ++kc) {
+    for (int kc = kc0; kc < kc1; ++kc) {
         __syncthreads();
         if constexpr (FUSED) {
             const ProjView vol = a.vol[0];
@@ -320,6 +328,29 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
             if ((tid & 31) == 0) sPn[(tid + s * NT) >> 5] = v;
         }
         __syncthreads();
+        // split-K: this slice's partial sums, in true units (the FP16 scales
+        // divided out), for the fixed-order float64 assembly pass
+        if (a.part_c) {
+            const size_t slice = (size_t)blockIdx.y;
+            for (int c = tid; c < TN; c += NT)
+                if (o0 + c < O) a.part_pn[slice * O + o0 + c] = sPn[c];
+#pragma unroll
+            for (int fa = 0; fa < FM; ++fa) {
+                const int T = g_T[fa];
+                const float inv = (FORM == kFP16) ? 1.f / (asc[fa] * bsc) : 1.f;
+#pragma unroll
+                for (int fb = 0; fb < FN; ++fb) {
+                    const int o = o0 + wn * FN * 16 + fb * 16 + r16;
+                    if (o >= O) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int t = g_t0[fa] + 4 * q + v;
+                        if (t < T) a.part_c[(slice * O + o) * T + t] = acc[fa][fb][v] * inv;
+                    }
+                }
+            }
+            return;
+        }
     }
 
     // ---- epilogue
@@ -750,7 +781,8 @@ static hipError_t launch_one(const CrossArgs &a, hipStream_t s)
     using Cfg = CrossCfg<FORM, WM, WN, FM, FN, FUSED>;
     const int total = a.n_mtiles * a.n_classes * a.ntiles_per_class;
     const int grid = ((total + 7) / 8) * 8;
-    hipLaunchKernelGGL((cross_kernel<FORM, WM, WN, FM, FN, FUSED>), dim3(grid),
+    const int slices = FUSED && a.ksplit > 1 ? a.ksplit : 1;
+    hipLaunchKernelGGL((cross_kernel<FORM, WM, WN, FM, FN, FUSED>), dim3(grid, slices),
                        dim3(Cfg::kThreads), 0, s, a);
     return hipGetLastError();
 }

@@ -49,7 +49,9 @@ struct PlanTables {
     std::vector<ProjView> views;
     std::vector<float> scales;
 };
-int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab);
+// forced_variant >= 0 picks that cross-term variant; -1 follows
+// rc_set_variant_override, then the autotuner's table, then the size rule.
+int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab, int forced_variant = -1);
 // Lay the plan's device buffers out from `mem` (NULL: only size them);
 // returns the bytes needed.
 size_t plan_bind(rc_plan *p, unsigned char *mem);
@@ -60,5 +62,7 @@ int plan_upload_async(rc_plan *p, const PlanTables &tab, unsigned char *staging,
 // Validate desc against the plan and launch the kernels on s.
 int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s);
 int validate_batch(const rc_batch_desc *d, bool need_ptrs);
+// rc_plan_create with an explicit variant (-1: the automatic choice).
+int plan_create_internal(const rc_batch_desc *d, int forced_variant, rc_plan **out);
 
 }  // namespace rc

@@ -93,6 +93,12 @@ struct CrossArgs {
     int n_mtiles;
     int persist_blocks;          // persistent variants: blocks in the grid
     int cc;                      // cross-correlation score instead of diff2
+    // split-K (fused single-image kernels only): blockIdx.y = slice s covers
+    // chunks [s * kc_per_split, min(nkc, (s + 1) * kc_per_split)) and writes
+    // its partial sums instead of diff2; ksplit <= 1 = one slice, normal epilogue
+    int ksplit, kc_per_split;
+    float *part_c;               // [ksplit][O][n_trans] cross-term partials
+    float *part_pn;              // [ksplit][O] sum corr/2 |P|^2 partials
     // outputs
     const float *diff2_init;     // same layout as diff2, may be NULL, may alias diff2
     float *diff2;
@@ -145,6 +151,30 @@ hipError_t launch_project_wide(int form, const ProjView *vols, const float *bsca
 // pnorm[i][c*O+o] = sum_p half_corr[i][p] * pabs2[c*O+o][p]
 hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,
                         const float *pabs2, float *pnorm, hipStream_t s);
+
+// Split-K assembly of one image: out[o*T+t] = score from the fixed-order
+// float64 sums over the S slices of part_c and part_pn (see CrossArgs).
+hipError_t launch_splitk_assemble(int O, int T, int S, const float *part_c, const float *part_pn,
+                                  const double *xnorm, const float *init, float *out, int cc,
+                                  hipStream_t s);
+
+// In-radius partition of n_images unfiltered images (one workgroup each).
+hipError_t launch_filter(int image_size, int n_images, int imgX, int imgY, int maxR, float pad,
+                         int maxR2_padded, const float *fimg, const float *corr, float *out_re,
+                         float *out_im, float *out_corr, float *coor_xy, int *filtered_size,
+                         hipStream_t s);
+
+// Single-image path (capi.cpp), shared with the allocation-free entry point:
+// workspace bytes of a shape and the run on a caller-provided workspace.
+size_t single_workspace_bytes(int T, int O, int npix, int form);
+int single_split(int T, int O, int npix, int form, int *ksplit, int *kc_per_split);
+int run_single_ws(int T, int O, int npix, int full_npix, int form, const float *eulers,
+                  const float *trans_x, const float *trans_y, const float *real, const float *imag,
+                  const rc_projector_kernel *pk, const float *corr, const float *init, float *dest,
+                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc);
+// Argument checks of rc_run_diff2_kernel_coarse.
+int check_drop_in(const rc_projector_kernel *pk, unsigned long O, unsigned long T,
+                  unsigned long image_size, int data_is_3D);
 
 // Per-particle min / argmin from the cross kernel's partials.
 hipError_t launch_argmin(int n_particles, const int *group_begin, int parts_per_group,

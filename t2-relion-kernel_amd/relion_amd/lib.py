@@ -36,7 +36,8 @@ EXPORTS = [
     "rc_server_create", "rc_server_submit", "rc_server_wait", "rc_server_diff2",
     "rc_server_release", "rc_server_stats", "rc_server_destroy", "rc_coarse_data_load",
     "rc_coarse_data_free", "rc_variant_name", "rc_coarse_weights",
-    "rc_run_diff2_kernel_fine",
+    "rc_run_diff2_kernel_fine", "rc_scratch_release", "rc_coarse_split",
+    "rc_filter_coarse_order", "rc_filter_coarse_images",
 ]
 
 
@@ -99,6 +100,11 @@ def load():
                                              i, i]
     L.rc_coarse_workspace_bytes.argtypes = [i, i, i, i, i]
     L.rc_coarse_workspace_bytes.restype = sz
+    L.rc_scratch_release.argtypes = [vp]
+    L.rc_coarse_split.argtypes = [i, i, i, i, C.POINTER(i), C.POINTER(i)]
+    L.rc_filter_coarse_order.argtypes = [C.POINTER(rc_projector_kernel), i, C.c_void_p,
+                                         C.POINTER(i)]
+    L.rc_filter_coarse_images.argtypes = [C.POINTER(rc_projector_kernel), i, i] + [vp] * 8
     L.rc_coarse_run.argtypes = [i, i, i, i, i, vp, vp, vp, vp, vp,
                                 C.POINTER(rc_projector_kernel), vp, vp, vp, vp, vp, vp, sz]
     L.rc_plan_create.argtypes = [C.POINTER(rc_batch_desc), C.POINTER(vp)]
@@ -220,6 +226,58 @@ class CoarseKernel:
                                    C.byref(pk), _ptr(corr), _ptr(diff2s), _ptr(diff2s_dest),
                                    _stream(stream), _ptr(coor_xy), _ptr(workspace),
                                    workspace.numel()))
+
+
+def coarse_split(translation_num, orientation_num, image_size, form="fp32"):
+    """(slices, 32-pixel chunks per slice) of the single-image calls' split-K."""
+    f = FORMS[form] if isinstance(form, str) else form
+    k, c = C.c_int(), C.c_int()
+    check(load().rc_coarse_split(translation_num, orientation_num, image_size, f, C.byref(k),
+                                 C.byref(c)))
+    return k.value, c.value
+
+
+def scratch_release(stream=None):
+    check(load().rc_scratch_release(_stream(stream)))
+
+
+def geometry_view(imgX, imgY, maxR, pad, maxR2_padded=None):
+    """A projector-kernel view holding only image geometry (no volume): enough
+    for the in-radius partition calls."""
+    k = rc_projector_kernel()
+    k.proj = None
+    k.imgX, k.imgY, k.imgZ, k.maxR = imgX, imgY, 1, maxR
+    k.padding_factor = pad
+    k.maxR2_padded = int(maxR * maxR * pad * pad) if maxR2_padded is None else maxR2_padded
+    return k
+
+
+def filter_order(view, image_size):
+    """rc_filter_coarse_order: (filtered position -> source pixel, inside count)."""
+    order = np.empty(image_size, np.int32)
+    n = C.c_int()
+    check(load().rc_filter_coarse_order(C.byref(view), int(image_size),
+                                        order.ctypes.data_as(C.c_void_p), C.byref(n)))
+    return order, n.value
+
+
+def filter_images(view, image_size, fimg, corr, stream=None):
+    """rc_filter_coarse_images on torch device tensors: fimg [n][2*image_size]
+    (real block, imaginary block), corr [n][image_size].  Returns real, imag,
+    corr [n][image_size], coor_xy [image_size*2], filtered size [n]."""
+    import torch
+    n = fimg.numel() // (2 * image_size)
+    dev = fimg.device
+    out = dict(real=torch.empty(n, image_size, dtype=torch.float32, device=dev),
+               imag=torch.empty(n, image_size, dtype=torch.float32, device=dev),
+               corr=torch.empty(n, image_size, dtype=torch.float32, device=dev),
+               coor_xy=torch.empty(2 * image_size, dtype=torch.float32, device=dev),
+               filtered_size=torch.empty(n, dtype=torch.int32, device=dev))
+    check(load().rc_filter_coarse_images(
+        C.byref(view), int(image_size), int(n), _ptr(fimg), _ptr(corr), _ptr(out["real"]),
+        _ptr(out["imag"]), _ptr(out["corr"]), _ptr(out["coor_xy"]), _ptr(out["filtered_size"]),
+        _stream(stream)))
+    return out
 
 
 def variants(form):

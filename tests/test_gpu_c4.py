@@ -23,7 +23,8 @@ def test_c4_reduced_against_oracle(torch_cuda):
     picks = [int(np.argmin(T)), int(np.argmax(T)), int(np.argsort(T)[len(T) // 2])]
     refs = {i: np.stack([oracle.problem_diff2(prob, i, cls=c) for c in range(C)]) for i in picks}
     best = {}
-    for form, (mx, mean) in (("bf16x3", (1e-5, 2e-6)), ("fp32", (2e-5, 1e-6))):
+    # about 5x the measured error (BF16x3 3.8e-7 / 4.7e-8 over these shapes)
+    for form, (mx, mean) in (("bf16x3", (1.9e-6, 2.5e-7)), ("fp32", (1.5e-6, 2e-7))):
         plan = lib.plan_for_problem(prob, projs, form)
         print(form, "variant", plan.tile())
         t = lib.problem_tensors(prob)

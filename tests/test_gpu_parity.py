@@ -4,13 +4,21 @@ Every test calls the gfx950 kernels through the C ABI (ctypes) and compares
 diff2 element-wise with ``oracle.diff2_f64`` using the reference's error
 metrics (check.cuh:96-111).  Tolerances (relative to the float64 value):
 
-* FP32 MFMA / FP32 direct forms: max 2e-5, mean 1e-6 (float32 projection,
-  phase and accumulation against float64; the reference's own float32
-  kernels sit at a mean of ~1e-7 against its float64 kernel);
-* BF16x3: max 1e-5, mean 2e-6, identical per-particle argmin;
-* FP16 (per-image scaled, one product): max 5e-3, mean 5e-4;
-* cross-correlation scores (which cross zero): max |error| <= 1e-5 (FP32) or
-  3e-5 (BF16x3) of the largest |score|.
+Each bound is about 5x the error measured on MI355X for that form and
+workload, so a tenfold precision regression fails:
+
+* 32^2 workload (8 particles x 576 x 21): FP32 MFMA max 1.2e-6 / mean
+  2.2e-7 (measured 2.4e-7 / 4.3e-8), BF16x3 3.5e-6 / 6e-7 (7e-7 / 1.2e-7),
+  FP16 (per-image scaled, one product) 2.5e-4 / 4e-5 (4.7e-5 / 7.4e-6),
+  FP32 direct form 7.5e-6 / 2.2e-6 (1.5e-6 / 4.5e-7);
+* ragged two-class batch (16^2 images of 32^3 maps at SNR 0.5, more
+  cancellation in the score): FP32 7.5e-6 / 1e-6 (1.4e-6 / 2.1e-7), BF16x3
+  2.8e-5 / 1e-5 (5.6e-6 / 2.0e-6);
+* FP16's arg-min must be a pose whose float64 score is within twice the
+  largest FP16 error of the float64 minimum (the agreement rate is printed);
+* cross-correlation scores (which cross zero): max |error| <= 7.5e-6 (FP32),
+  2e-5 (BF16x3) or 1.5e-3 (FP16) of the largest |score| (measured 1.5e-6,
+  4e-6, 2.9e-4).
 """
 import numpy as np
 import pytest
@@ -19,8 +27,10 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": (2e-5, 1e-6), "fp32_direct": (2e-5, 1e-6), "bf16x3": (1e-5, 2e-6),
-       "fp16": (5e-3, 5e-4)}
+TOL = {"fp32": (1.2e-6, 2.2e-7), "fp32_direct": (7.5e-6, 2.2e-6), "bf16x3": (3.5e-6, 6e-7),
+       "fp16": (2.5e-4, 4e-5)}
+RAGGED_TOL = {"fp32": (7.5e-6, 1e-6), "bf16x3": (2.8e-5, 1e-5)}
+CC_TOL = {"fp32": 7.5e-6, "bf16x3": 2e-5, "fp16": 1.5e-3}
 
 
 @pytest.fixture(scope="module")
@@ -55,9 +65,9 @@ def _assert_close_scaled(got, ref, tol, what):
     assert err <= tol, (what, err)
 
 
-def _assert_close(got, ref, form, what):
+def _assert_close(got, ref, form, what, tol=None):
     st = oracle.error_stats(got, ref)
-    mx, mean = TOL[form]
+    mx, mean = (tol or TOL)[form]
     print(f"{what} [{form}] max_rel={st['max_rel']:.3e} mean_rel={st['mean_rel']:.3e}")
     assert st["max_rel"] <= mx and st["mean_rel"] <= mean, (what, form, st)
 
@@ -163,8 +173,13 @@ def _wide_256(lib, form):
     return [v for v, tm, tn in lib.variants(form) if tm == 256 and tn == 256]
 
 
-@pytest.mark.parametrize("kernel", ["default", "wide"])
-@pytest.mark.parametrize("form", ["fp32", "bf16x3"])
+def _wide_512(lib, form):
+    """Indices of the 128 x 512 wide kernels (the large-batch default)."""
+    return [v for v, tm, tn in lib.variants(form) if tm == 128 and tn == 512]
+
+
+@pytest.mark.parametrize("kernel", ["default", "wide", "wide512"])
+@pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
 def test_batched_cc(c1, lib, torch_cuda, form, kernel):
     """Batched cross-correlation scores; the per-particle minimum is the best
     correlation."""
@@ -180,6 +195,8 @@ def test_batched_cc(c1, lib, torch_cuda, form, kernel):
     try:
         if kernel == "wide":
             lib.set_variant_override(_wide_256(lib, form)[0])
+        elif kernel == "wide512":
+            lib.set_variant_override(_wide_512(lib, form)[-1])
         plan = lib.plan_for_problem(prob, projs, form, cc=True)
         t = lib.problem_tensors(prob)
         plan.execute(t)
@@ -193,9 +210,10 @@ def test_batched_cc(c1, lib, torch_cuda, form, kernel):
         tx, ty = prob.trans_of(i)
         ref = oracle.cc_f64(prob.vols[0], g.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers,
                             tx, ty, pb.real[i][inv], pb.imag[i][inv], pb.corr[i][inv])
-        _assert_close_scaled(d[off[i]:off[i + 1]], ref.ravel(), 1e-5 if form == "fp32" else 3e-5,
-                             f"batched cc {i}")
-        assert bidx[i] == oracle.argmin_flat(ref)
+        _assert_close_scaled(d[off[i]:off[i + 1]], ref.ravel(), CC_TOL[form],
+                             f"batched cc {i} ({kernel})")
+        if form != "fp16":
+            assert bidx[i] == oracle.argmin_flat(ref)
 
 
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
@@ -211,14 +229,22 @@ def test_batched_c1(c1, lib, torch_cuda, form):
     off = lib.output_offsets(prob.trans_offset, prob.n_classes, prob.orientation_num)
     best = t["best_diff2"].cpu().numpy()
     bidx = t["best_index"].cpu().numpy()
+    agree = 0
     for i in range(prob.n_particles):
         got = d[off[i]:off[i + 1]]
         _assert_close(got, ref[i].ravel(), form, f"batched particle {i}")
+        r = ref[i].ravel()
         k = oracle.argmin_flat(ref[i])
+        agree += int(bidx[i] == k)
         if form != "fp16":
             assert bidx[i] == k, (i, bidx[i], k)
+        else:
+            # a near-optimal pose: its float64 score is within twice the
+            # largest FP16 error of the float64 minimum
+            assert r[bidx[i]] - r[k] <= 2.0 * np.abs(got - r).max(), (i, bidx[i], k)
         assert abs(best[i] - got[bidx[i]]) <= 1e-6 * abs(best[i])
         assert got[bidx[i]] == got.min()
+    print(f"[{form}] arg-min agreement with float64: {agree}/{prob.n_particles}")
 
 
 def _ragged_problem():
@@ -259,7 +285,7 @@ def test_batched_ragged_multiclass(lib, torch_cuda):
                 bidx = t["best_index"].cpu().numpy()
                 for i in range(prob.n_particles):
                     _assert_close(d[off[i]:off[i + 1]], refs[i].ravel(), form,
-                                  f"ragged variant {v} ({tm}x{tn}) particle {i}")
+                                  f"ragged variant {v} ({tm}x{tn}) particle {i}", RAGGED_TOL)
                     assert bidx[i] == oracle.argmin_flat(refs[i]), (form, v, i)
                 plan.close()
     finally:

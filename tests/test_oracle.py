@@ -316,3 +316,33 @@ def test_cc_score_against_diff2_expansion():
         pn = np.sum(wu / 2.0 * np.abs(P) ** 2)
         want = -(pn + xn - d2[o]) / np.sqrt(2.0 * pn)
         np.testing.assert_allclose(cc[o], want, rtol=1e-9, atol=1e-12 * np.abs(want).max())
+
+
+def test_expanded_form_equals_direct_form():
+    """The expanded-form evaluator (projections once, float64 matrix products)
+    used for the full-size GPU checks gives the direct form's values: every
+    C1 particle, and a ragged two-class problem with an out-of-radius tail."""
+    prob = synth.make_problem("c1", seed=17)
+    d = oracle.problem_diff2_expanded(prob, range(prob.n_particles))
+    for i in range(prob.n_particles):
+        ref = oracle.problem_diff2(prob, i)
+        assert np.abs(d[i] - ref).max() <= 1e-12 * np.abs(ref).max()
+        assert oracle.argmin_flat(d[i]) == oracle.argmin_flat(ref)
+    vols = [synth.fourier_volume(synth.blob_map(32, seed=s)) for s in (3, 4)]
+    g = synth.CoarseGeometry(imgY=16, maxR=6)            # many pixels outside the radius
+    eul = synth.orientation_set(0, 8)
+    pb = synth.make_particles(vols, g, 3, 32, 0.5, seed=9, shift_range=1.0)
+    assert pb.full_npix > pb.npix
+    txs, tys, offs = [], [], [0]
+    for i, r2 in enumerate((1, 13, 4)):
+        tx, ty, _ = synth.translation_set(r2, 32, centre=(0.5 * i, -0.25 * i))
+        txs.append(tx)
+        tys.append(ty)
+        offs.append(offs[-1] + tx.size)
+    p2 = synth.CoarseProblem("ragged", vols, g, eul, np.asarray(offs, np.int32),
+                             np.concatenate(txs), np.concatenate(tys), pb, 32)
+    for c in range(2):
+        d = oracle.problem_diff2_expanded(p2, range(3), cls=c)
+        for i in range(3):
+            ref = oracle.problem_diff2(p2, i, cls=c)
+            assert np.abs(d[i] - ref).max() <= 1e-12 * np.abs(ref).max()
