@@ -120,6 +120,24 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
                                unsigned long image_size, void *stream, int do_CC,
                                int data_is_3D);
 
+/* rc_run_diff2_kernel_coarse without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops).  Same arguments, semantics, kernels
+ * and results (bit-identical) as rc_run_diff2_kernel_coarse; its scratch is
+ * a 40 MiB device array inside the library (one per GPU, held until the
+ * process ends), so shapes whose workspace (rc_coarse_workspace_bytes with
+ * the FP32 form) exceeds 40 MiB are rejected with RC_ERR_LIMIT before any
+ * launch.  Calls that overlap on one GPU -- any two streams, or host threads
+ * -- share that array and race: serialise them. */
+int rc_run_diff2_kernel_coarse_noalloc(const rc_projector_kernel *projector,
+                                       const float *d_trans_x, const float *d_trans_y,
+                                       const float *d_trans_z, const float *d_corr_img,
+                                       const float *d_fimg_real, const float *d_fimg_imag,
+                                       const float *d_eulers, float *d_diff2s,
+                                       float local_sqrtXi2, unsigned long orientation_num,
+                                       unsigned long translation_num, unsigned long image_size,
+                                       void *stream, int do_CC, int data_is_3D);
+
 /* Frees the scratch rc_run_diff2_kernel_coarse keeps for `stream` on the
  * current device (synchronises that stream first).  No counterpart in the
  * reference, whose kernels keep no state. */

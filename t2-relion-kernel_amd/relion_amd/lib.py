@@ -37,7 +37,7 @@ EXPORTS = [
     "rc_server_release", "rc_server_stats", "rc_server_destroy", "rc_coarse_data_load",
     "rc_coarse_data_free", "rc_variant_name", "rc_coarse_weights",
     "rc_run_diff2_kernel_fine", "rc_scratch_release", "rc_coarse_split",
-    "rc_filter_coarse_order", "rc_filter_coarse_images",
+    "rc_filter_coarse_order", "rc_filter_coarse_images", "rc_run_diff2_kernel_coarse_noalloc",
 ]
 
 
@@ -100,6 +100,7 @@ def load():
                                              i, i]
     L.rc_coarse_workspace_bytes.argtypes = [i, i, i, i, i]
     L.rc_coarse_workspace_bytes.restype = sz
+    L.rc_run_diff2_kernel_coarse_noalloc.argtypes = L.rc_run_diff2_kernel_coarse.argtypes
     L.rc_scratch_release.argtypes = [vp]
     L.rc_coarse_split.argtypes = [i, i, i, i, C.POINTER(i), C.POINTER(i)]
     L.rc_filter_coarse_order.argtypes = [C.POINTER(rc_projector_kernel), i, C.c_void_p,
@@ -196,6 +197,17 @@ def run_diff2_kernel_coarse(pk, trans_x, trans_y, corr, real, imag, eulers, diff
                             do_CC=False, data_is_3D=False):
     """Mirror of runDiff2KernelCoarse: ``diff2s[o*T+t] +=`` score (torch tensors)."""
     check(load().rc_run_diff2_kernel_coarse(
+        C.byref(pk), _ptr(trans_x), _ptr(trans_y), None, _ptr(corr), _ptr(real), _ptr(imag),
+        _ptr(eulers), _ptr(diff2s), 1.0, orientation_num, translation_num, image_size,
+        _stream(stream), int(do_CC), int(data_is_3D)))
+
+
+def run_diff2_kernel_coarse_noalloc(pk, trans_x, trans_y, corr, real, imag, eulers, diff2s,
+                                    orientation_num, translation_num, image_size, stream=None,
+                                    do_CC=False, data_is_3D=False):
+    """rc_run_diff2_kernel_coarse_noalloc: the same call with the library's
+    static scratch (no allocation; overlapping calls on one GPU race)."""
+    check(load().rc_run_diff2_kernel_coarse_noalloc(
         C.byref(pk), _ptr(trans_x), _ptr(trans_y), None, _ptr(corr), _ptr(real), _ptr(imag),
         _ptr(eulers), _ptr(diff2s), 1.0, orientation_num, translation_num, image_size,
         _stream(stream), int(do_CC), int(data_is_3D)))
