@@ -73,13 +73,34 @@ def measured_traffic(kernel, form, prob, npix):
     return (b, rel) if b else (None, None)
 
 
+def host_cpu():
+    """(threads this job may use, nproc, CPU model) of the host."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS")
+    threads = min(avail, int(env)) if env and env.isdigit() and int(env) > 0 else avail
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return threads, os.cpu_count() or 1, model
+
+
 def cpu_baseline(prob, target_s):
-    """float32 port of the CUDA-core loop on this host's cores, on whole
-    particles of the same workload until about target_s seconds pass."""
+    """float32 port of the CUDA-core loop on the host cores this job may use
+    (its CPU affinity, capped by OMP_NUM_THREADS), on whole particles of the
+    same workload until about target_s seconds pass."""
     from oracle import oracle
     pb = prob.particles
-    threads = min(16, os.cpu_count() or 1)
+    threads, nproc, model = host_cpu()
     evals, t_used, n = 0, 0.0, 0
+    used = threads
     while n < prob.n_particles and (t_used < target_s or n == 0):
         tx, ty = prob.trans_of(n)
         t0 = time.perf_counter()
@@ -90,56 +111,223 @@ def cpu_baseline(prob, target_s):
         evals += tx.size * prob.orientation_num
         n += 1
     return dict(value=evals / t_used, unit="diff2 evals/s", cores=used, kind="port",
+                nproc=nproc, cpu_model=model,
                 sample=f"{n} particle(s) x {prob.orientation_num} orientations x "
                        f"{int(prob.trans_of(0)[0].size)} translations x {pb.full_npix} pixels, "
-                       f"{t_used:.1f} s, float32 port of cuda_kernel_diff2_coarse with OpenMP")
+                       f"{t_used:.1f} s, float32 port of cuda_kernel_diff2_coarse with OpenMP "
+                       f"on {used} of {nproc} host threads ({model})")
 
 
-def fp64_check(lib, torch, prob, outs, n_orient):
-    """Error of each form against the float64 oracle on particle 0 (first
-    n_orient orientations), and argmin agreement between the forms."""
+def fp64_check(prob, outs, picks, lib):
+    """Each form against the float64 oracle (expanded form) over all
+    orientations and translations of the particles `picks`: max / mean
+    relative error (the reference's metric), arg-min agreement with float64
+    and, for each disagreement, the float64 top-2 margin; plus the forms'
+    arg-min agreement over the whole batch."""
     import numpy as np
     from oracle import oracle
-    ref = oracle.problem_diff2(prob, 0, orient=np.arange(n_orient)).ravel()
-    T = int(prob.trans_of(0)[0].size)
-    rep = {}
+    refs = oracle.problem_diff2_expanded(prob, picks)
+    off = lib.output_offsets(prob.trans_offset, prob.n_classes, prob.orientation_num)
+    rep = {"particles": len(picks), "orientations": prob.orientation_num}
     for form, t in outs.items():
-        d = t["diff2"][: n_orient * T].double().cpu().numpy()
-        st = oracle.error_stats(d, ref)
-        rep[form] = dict(max_rel=st["max_rel"], mean_rel=st["mean_rel"])
+        mx, means, agree, misses = 0.0, [], 0, []
+        best = t["best_index"].cpu().numpy()
+        for i in picks:
+            got = t["diff2"][off[i]:off[i + 1]].double().cpu().numpy()
+            ref = refs[i].ravel()
+            st = oracle.error_stats(got, ref)
+            mx = max(mx, st["max_rel"])
+            means.append(st["mean_rel"])
+            k = oracle.argmin_flat(ref)
+            if int(best[i]) == k:
+                agree += 1
+            else:
+                r2 = np.sort(ref)[:2]
+                misses.append(dict(particle=int(i), top2_margin=float((r2[1] - r2[0]) / abs(r2[0])),
+                                   value_gap=float((ref[best[i]] - ref[k]) / abs(ref[k]))))
+        rep[form] = dict(max_rel=mx, mean_rel=float(np.mean(means)),
+                         argmin_agree_fp64=agree / len(picks), argmin_misses=misses)
     forms = list(outs)
     if len(forms) > 1:
         a = outs[forms[0]]["best_index"].cpu().numpy()
         for f in forms[1:]:
             b = outs[f]["best_index"].cpu().numpy()
-            rep[f"argmin_agree_{f}_vs_{forms[0]}"] = float((a == b).mean())
+            rep[f"argmin_agree_{f}_vs_{forms[0]}_all_particles"] = float((a == b).mean())
     return rep
 
 
-def serve_bench(n_requests, rate_hz, forms, slots, budget_gb):
+def event_ms(torch, fn, n, stream=None):
+    """Mean milliseconds of fn() over n back-to-back calls (HIP events on
+    the launch stream, after two untimed calls)."""
+    s = stream or torch.cuda.current_stream()
+    for _ in range(2):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(n):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / n
+
+
+def single_image_bench(lib, torch, cpu_seconds):
+    """Per-call latency of the single-image drop-in (rc_run_diff2_kernel_coarse,
+    unfiltered image, diff2 += score) at the 32^2 workload (21 x 576 x 544 px)
+    and at one production particle (113 x 4608 x 2112 px), with the float32
+    CPU port timed on the same image."""
+    import numpy as np
+    from relion_amd import synth
+    out = {}
+    threads, _, _ = host_cpu()
+    for name, cfg in (("c1_particle", "c1"), ("c2_particle", "c2")):
+        prob = synth.make_problem(cfg, particles=1, seed=3)
+        g, pb = prob.geom, prob.particles
+        order, _ = synth.filter_order(g)
+        inv = np.empty_like(order)
+        inv[order] = np.arange(order.size)
+        proj = lib.Projector(prob.vols[0])
+        pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+        tx, ty = prob.trans_of(0)
+        O, T = prob.orientation_num, tx.size
+        d = lambda a: torch.as_tensor(np.ascontiguousarray(a)).cuda()
+        ins = [d(tx), d(ty), d(pb.corr[0][inv]), d(pb.real[0][inv]), d(pb.imag[0][inv]),
+               d(prob.eulers.ravel())]
+        diff2 = torch.zeros(O * T, dtype=torch.float32, device="cuda")
+        call = lambda: lib.run_diff2_kernel_coarse(pk, *ins[:5], ins[5], diff2, O, T,
+                                                   g.image_size)
+        ms = event_ms(torch, call, 50)
+        ks, kc = lib.coarse_split(T, O, g.image_size)
+        rec = dict(workload=f"{T} translations x {O} orientations x {g.image_size} px "
+                            f"(unfiltered, {pb.npix} in radius)",
+                   us_per_call=ms * 1e3, evals_per_s=O * T / (ms * 1e-3), split_slices=ks,
+                   chunks_per_slice=kc)
+        if cpu_seconds > 0:
+            from oracle import oracle
+            t0 = time.perf_counter()
+            n = 0
+            while n == 0 or (time.perf_counter() - t0 < cpu_seconds and n < 50):
+                oracle.diff2_f32(prob.vols[0], g.maxR2_padded, prob.eulers, tx, ty, pb.real[0],
+                                 pb.imag[0], pb.corr[0], pb.coor_xy, pb.npix, threads=threads)
+                n += 1
+            dt = (time.perf_counter() - t0) / n
+            rec["cpu_port_us_per_call"] = dt * 1e6
+            rec["cpu_port_threads"] = threads
+        out[name] = rec
+        proj.close()
+        log(f"single image {name}: {rec}")
+    return out
+
+
+def c4_bench(lib, torch, steps, warmup, tune):
+    """The 8-class batch (2048 particles, 49-197 translations each, 4608
+    orientations, 1636 px): evals/s and the cross term's TFLOP/s per form."""
+    from relion_amd import synth
+    t0 = time.perf_counter()
+    prob = synth.make_problem("c4", seed=21)
+    log(f"generated c4: {prob.n_particles} particles x {prob.n_classes} classes, "
+        f"{int(prob.trans_offset[-1])} translations in {time.perf_counter() - t0:.1f}s")
+    projs = [lib.Projector(v) for v in prob.vols]
+    evals = prob.evals()
+    flops = 4.0 * prob.particles.npix * evals
+    out = dict(workload=f"{prob.n_particles} particles x {prob.n_classes} classes x "
+                        f"{prob.orientation_num} orientations, 49-197 translations "
+                        f"({int(prob.trans_offset[-1])} in all) x {prob.particles.npix} px",
+               evals_per_step=evals, flops_per_step=flops)
+    t = lib.problem_tensors(prob)
+    for form in ("bf16x3", "fp32"):
+        tuned = None
+        if tune and form == "bf16x3":
+            probe = lib.plan_for_problem(prob, projs, form)
+            tuned = probe.autotune(t, iters=2)
+            probe.close()
+        plan = lib.plan_for_problem(prob, projs, form)
+        plan.timing(max(steps, 1))
+        for _ in range(warmup):
+            plan.execute(t)
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(steps):
+            plan.execute(t)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - a) * 1e3 / steps
+        cross = plan.timing_read(steps)
+        cross_ms = sum(cross) / len(cross)
+        peak = PEAK_TFLOPS[form]
+        out[form] = dict(ms_per_step=ms, evals_per_s=evals / (ms * 1e-3), cross_ms=cross_ms,
+                         cross_tflops=flops / (cross_ms * 1e-3) / 1e12,
+                         frac_of_peak=flops / (cross_ms * 1e-3) / 1e12 / peak,
+                         variant=plan.tile(), kernel=lib.variant_name(form, plan.tile()[0]),
+                         autotuned=tuned is not None)
+        plan.close()
+        log(f"c4 {form}: {out[form]}")
+    del t
+    for p in projs:
+        p.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def c3_bench(lib, torch):
+    """Precision forms on the 32^2 workload (8 particles x 576 x 21):
+    max / mean relative error against float64 and arg-min agreement."""
+    import numpy as np
+    from oracle import oracle
+    from relion_amd import synth
+    prob = synth.make_problem("c1", seed=11)
+    projs = [lib.Projector(v) for v in prob.vols]
+    refs = [oracle.problem_diff2(prob, i).ravel() for i in range(prob.n_particles)]
+    off = lib.output_offsets(prob.trans_offset, 1, prob.orientation_num)
+    out = dict(workload="8 particles x 576 orientations x 21 translations x 414 px")
+    for form in ("fp32", "bf16x3", "fp16"):
+        plan = lib.plan_for_problem(prob, projs, form)
+        t = lib.problem_tensors(prob)
+        plan.execute(t)
+        torch.cuda.synchronize()
+        d = t["diff2"].double().cpu().numpy()
+        best = t["best_index"].cpu().numpy()
+        mx, means, agree = 0.0, [], 0
+        for i in range(prob.n_particles):
+            st = oracle.error_stats(d[off[i]:off[i + 1]], refs[i])
+            mx = max(mx, st["max_rel"])
+            means.append(st["mean_rel"])
+            agree += int(best[i] == oracle.argmin_flat(refs[i]))
+        out[form] = dict(max_rel=mx, mean_rel=float(np.mean(means)),
+                         argmin_agree_fp64=agree / prob.n_particles)
+        plan.close()
+    log(f"c3 precision: {out}")
+    return out
+
+
+def serve_bench(n_requests, rates, forms, slots, budget_gb, pool):
     """Streamed serving: Poisson arrivals of mixed-box micrograph requests,
     continuous batching on one GPU; per-request latency from submission to
-    result, and aggregate evals/s, for each form on the same stream."""
+    result, and aggregate evals/s, for each form and offered rate.  The
+    requests are drawn from a pool of `pool` generated ones (their inputs
+    are re-sent in full each time)."""
     from relion_amd import serving
     t0 = time.perf_counter()
-    stream, cache = serving.make_stream(n_requests, rate_hz)
-    log(f"serving stream: {n_requests} requests at {rate_hz}/s generated in "
-        f"{time.perf_counter() - t0:.1f}s")
+    base, cache = serving.make_stream(pool, 1.0)
+    log(f"serving pool: {pool} requests generated in {time.perf_counter() - t0:.1f}s")
     out = {}
-    for form in forms:
-        srv = serving.Server(form, hbm_budget=int(budget_gb * (1 << 30)),
-                             pinned_budget=4 << 30, max_inflight=slots)
-        # warm-up request (kernel loading) outside the measured stream
-        tk = srv.submit(stream[0][1])
-        srv.wait(tk, stream[0][1].P)
-        lat, wall = serving.run_stream(srv, stream)
-        rep = serving.summarize(lat, stream, wall)
-        rep["server"] = srv.stats()
-        srv.close()
-        out[form] = rep
-        log(f"serve {form}: p50 {rep['p50_ms']:.2f} ms p99 {rep['p99_ms']:.2f} ms, "
-            f"{rep['evals_per_s']:.3e} evals/s, peak inflight {rep['server']['peak_inflight']}")
-    out["workload"] = (f"{n_requests} requests, Poisson {rate_hz}/s, 20-400 particles, "
+    for rate in rates:
+        stream = serving.resample_stream(base, n_requests, rate, seed=int(rate))
+        for form in forms:
+            srv = serving.Server(form, hbm_budget=int(budget_gb * (1 << 30)),
+                                 pinned_budget=4 << 30, max_inflight=slots)
+            tk = srv.submit(stream[0][1])            # kernel loading, outside the stream
+            srv.wait(tk, stream[0][1].P)
+            lat, wall = serving.run_stream(srv, stream)
+            rep = serving.summarize(lat, stream, wall)
+            rep["server"] = srv.stats()
+            srv.close()
+            out[f"{form}@{rate:g}Hz"] = rep
+            log(f"serve {form} at {rate:g}/s: p50 {rep['p50_ms']:.2f} ms p99 "
+                f"{rep['p99_ms']:.2f} ms, {rep['evals_per_s']:.3e} evals/s, peak inflight "
+                f"{rep['server']['peak_inflight']}")
+    out["workload"] = (f"{n_requests} requests per rate (drawn from {pool} generated), Poisson "
+                       f"arrivals at {', '.join(f'{r:g}' for r in rates)}/s, 20-400 particles, "
                        "box {64,128,256} -> coarse {32,64,96}, 1-4 classes, 4608 orientations, "
                        f"shift radius 3-6 px; {slots} slots, HBM budget {budget_gb} GiB")
     return out
@@ -241,7 +429,7 @@ def downstream_bench(lib, torch, prob, t, steps, cpu_seconds):
         dt = time.perf_counter() - t0
         fine["cpu_port_evals_per_s"] = 4 * take / dt
         fine["cpu_port_sample"] = (f"all {take} jobs, float64 oracle with OpenMP on "
-                                   f"{min(16, os.cpu_count() or 1)} host threads, {dt:.2f} s")
+                                   f"{host_cpu()[0]} host threads, {dt:.2f} s")
     out["fine"] = fine
     proj.close()
     return out
@@ -252,22 +440,28 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     # The first ~10 steps of a freshly created plan run up to 8 % slower than
-    # the steady state (the step-time log shows them settling); the default
-    # warm-up covers them.
+    # the steady state (the clock ramps up under load; the step-time log
+    # shows it settling); the default warm-up covers them.
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--config", default="c2")
     ap.add_argument("--particles", type=int, default=None)
     ap.add_argument("--forms", default="bf16x3,fp32")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--check-orient", type=int, default=256)
+    ap.add_argument("--check-particles", type=int, default=16,
+                    help="particles checked against float64 over all orientations")
     ap.add_argument("--no-tune", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=40)
-    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rates", default="40,400")
+    ap.add_argument("--serve-pool", type=int, default=40)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-budget-gb", type=float, default=48.0)
     ap.add_argument("--no-downstream", action="store_true",
                     help="skip timing the weights/significance and fine-pass steps")
+    ap.add_argument("--no-single", action="store_true",
+                    help="skip the single-image drop-in latency leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the precision-form leg")
+    ap.add_argument("--no-c4", action="store_true", help="skip the 8-class batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; particle batches would "
@@ -275,6 +469,7 @@ def main():
                                    "which is not built", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import numpy as np
     import torch
     from relion_amd import lib, synth
 
@@ -300,22 +495,33 @@ def main():
         log(f"{form}: {r['ms_per_step']:.3f} ms/step, cross {r['cross_ms']:.3f} ms, "
             f"{r['evals_per_s']:.3e} evals/s, {r['cross_tflops']:.1f} TFLOP/s, tile {r['variant']}")
 
-    streamed = None
-    if args.serve_requests > 0:
-        streamed = serve_bench(args.serve_requests, args.serve_rate, forms, args.serve_slots,
-                               args.serve_budget_gb)
-
-    check = fp64_check(lib, torch, prob, outs, min(args.check_orient, prob.orientation_num))
+    n_chk = max(1, min(args.check_particles, prob.n_particles))
+    picks = sorted(set(np.linspace(0, prob.n_particles - 1, n_chk).astype(int).tolist()))
+    check = fp64_check(prob, outs, picks, lib)
     log(f"fp64 check: {check}")
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(prob, args.cpu_seconds)
-        log(f"cpu baseline: {cpu}")
     downstream = None
     if not args.no_downstream:
         downstream = downstream_bench(lib, torch, prob, outs[forms[0]], args.steps,
                                       0.0 if args.no_cpu_baseline else 5.0)
         log(f"downstream: {downstream}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(prob, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+    del outs
+    for p in projs:
+        p.close()
+    torch.cuda.empty_cache()
+    single = None if args.no_single else single_image_bench(
+        lib, torch, 0.0 if args.no_cpu_baseline else 3.0)
+    c3 = None if args.no_c3 else c3_bench(lib, torch)
+    c4 = None if args.no_c4 else c4_bench(lib, torch, max(3, args.steps // 2), 3,
+                                           not args.no_tune)
+    streamed = None
+    if args.serve_requests > 0:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        streamed = serve_bench(args.serve_requests, rates, forms, args.serve_slots,
+                               args.serve_budget_gb, args.serve_pool)
 
     head = forms[0]
     r = results[head]
@@ -344,9 +550,15 @@ def main():
                      "unit": "TFLOP/s", "frac": r["cross_tflops"] / peak, "traffic": traffic,
                      "traffic_source": traffic_src,
                      "kernel": r["kernel"],
+                     "timed": "HIP events around the cross term's launches: the translation "
+                              "operand precompute (encode_a_kernel, schedule-8 variants) and "
+                              "the cross kernel",
                      "flops_per_launch": 4.0 * npix * evals},
         "forms": {f: {k: v for k, v in x.items()} for f, x in results.items()},
         "precision_vs_fp64": check,
+        "precision_c3": c3,
+        "single_image": single,
+        "classification_c4": c4,
         "streamed": streamed,
         "downstream": downstream,
         "cpu_baseline": cpu,

@@ -382,24 +382,27 @@ static TuneKey tune_key(const rc_batch_desc *d)
 
 namespace rc {
 
-// Variant used when the autotuner has not seen a shape: the fastest of the
-// compiled table at 512 x 4608 x 113 x 1636 on MI355X (pipelined 128 x 128
-// for FP32 and BF16x3, pipelined 64 x 256 with one row fragment per wave
-// for FP16).
-// Measured on MI355X (512 particles x 113 translations x 4608 orientations x
-// 1636 pixels): the wide 128 x 512 kernel with fenced segments is the
-// fastest of every form once its tiles fill the device and the 512-column
-// tiles waste little; then the wide staggered 256 x 256 one; smaller
-// searches use the pipelined 128 x 128 kernel (FP16: 64 x 256), whose finer
-// tiles waste less padding.  rc_autotune overrides this per shape.
+// Variant used when the autotuner has not seen a shape.  Measured on MI355X
+// at 512 particles x 113 translations x 4608 orientations x 1636 pixels:
+// BF16X3 and FP16 run fastest on the wide 128 x 512 kernel with fenced
+// segments (translation operand built in the kernel, 4.0-4.1 ms); FP32 on
+// the wide 256 x 256 kernel with the translation operand precomputed (12.4
+// vs 13.2 ms: its longer MFMA phase hides the operand copies, while the
+// VALU build competes with its FP32 MFMAs).  These need enough tiles to fill
+// the device and, for 512 columns, little column waste; then the wide
+// 256 x 256 kernel with staggered waves; smaller searches use the pipelined
+// 128 x 128 kernel (FP16: 64 x 256), whose finer tiles waste less padding.
+// rc_autotune overrides this per shape.
 static int default_variant(int form, int n_rows, int n_classes, int O)
 {
-    constexpr int kWide256 = 14;   // wide, 256 x 256, staggered waves
-    constexpr int kWide512 = 21;   // wide, 128 x 512, fenced segments
+    constexpr int kWide256 = 14;     // wide, 256 x 256, staggered waves
+    constexpr int kWide512 = 21;     // wide, 128 x 512, fenced segments
+    constexpr int kWide256Pre = 23;  // wide, 256 x 256, translation operand precomputed
     const long long rows = n_rows;   // the wide kernels pack the rows
     const long long t512 = ((rows + 127) / 128) * n_classes * ((O + 511) / 512);
     const long long t256 = ((rows + 255) / 256) * n_classes * ((O + 255) / 256);
     const bool narrow_waste = (long long)((O + 511) / 512) * 512 > O + O / 8;
+    if (form == kFP32 && t256 >= 1024) return kWide256Pre;
     if (t512 >= 1024 && !narrow_waste) return kWide512;
     if (t256 >= 512) return kWide256;
     return form == kFP16 ? 7 : 4;
@@ -455,6 +458,13 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     const size_t prows = p->packed ? (size_t)p->n_rows : (size_t)p->n_groups;
     p->part_min = c.take<double>(prows * nt_total * p->parts_per_group);
     p->part_idx = c.take<long long>(prows * nt_total * p->parts_per_group);
+    // schedule 8: the translation operand of every row tile and chunk, in
+    // the LDS image of the kernel's A buffer (tile_m rows x row bytes)
+    p->a_chunks = (p->npix + 15) / 16;
+    p->a_enc = nullptr;
+    if (p->info.wide && p->info.schedule == 8)
+        p->a_enc = c.take<unsigned char>((size_t)p->n_mtiles * p->a_chunks * p->info.tile_m *
+                                         (p->form == kFP16 ? 64 : 128));
     return align_up(c.off, 256);
 }
 
@@ -610,8 +620,14 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
     a.part_idx = want_best ? p->part_idx : nullptr;
     a.persist_blocks = p->persist_blocks;
     a.cc = d->do_cc ? 1 : 0;
+    a.a_enc = p->a_enc;
+    a.a_chunks = p->a_chunks;
+    // the timing slots bracket the whole cross term: the translation
+    // operand's precompute (schedule 8) and the cross kernel
     const int nslot = p->capturing ? 0 : (int)p->ev.size() / 2;
     if (nslot) RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot)], s));
+    if (p->a_enc)
+        RC_HIP(launch_encode_a(p->form, p->info.tile_m, a, p->a_enc, s));
     RC_HIP(launch_cross(p->form, false, p->variant, a, s));
     if (nslot) {
         RC_HIP(hipEventRecord(p->ev[2 * (p->launches % nslot) + 1], s));
@@ -1041,6 +1057,7 @@ const char *rc_variant_name(int form, int variant)
              : v->schedule == 1 ? ", staggered waves (bit 2)"
              : v->schedule == 2 ? ", staggered waves (bit 0)"
              : v->schedule == 4 ? ", fenced segments"
+             : v->schedule == 8 ? ", translation operand precomputed"
                                 : "");
     return buf;
 }

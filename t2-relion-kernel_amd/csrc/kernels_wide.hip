@@ -269,6 +269,69 @@ __device__ __forceinline__ void a_build(const ARow &ar, const float4 (&xw)[PPT /
     a_store<FORM, PPT>(ar, v, h, r, dst);
 }
 
+// The A row of packed row R for a thread building pixels PPT*gh ..
+// PPT*gh+PPT-1 of each chunk.  Rows past the batch's last one are built from
+// particle 0 with zero shift (finite, never stored or ranked).
+template <int FORM, int PPT>
+__device__ __forceinline__ ARow make_arow(const CrossArgs &a, int R, int gh)
+{
+    ARow ar;
+    const bool ok = R < a.n_rows;
+    const int part = ok ? a.row_part[R] : 0;
+    ar.txr = ok ? (double)a.trans_x[R] * kInv2PiW : 0.0;
+    ar.tyr = ok ? (double)a.trans_y[R] * kInv2PiW : 0.0;
+    ar.scale = (FORM == kFP16) ? a.ascale[part] : 1.f;
+    ar.xw = a.xw + (size_t)part * a.kpad + PPT * gh;
+    return ar;
+}
+
+// ---------------------------------------------------------------------------
+// precomputed translation operand (schedule 8): block = one row tile x one
+// 16-pixel chunk, built exactly as the in-kernel build does into an LDS image
+// of the tile's A buffer, then written out as one contiguous block
+// ---------------------------------------------------------------------------
+template <int FORM, int TM, int NT>
+__global__ __launch_bounds__(NT) void encode_a_kernel(const CrossArgs a,
+                                                      unsigned char *__restrict__ out)
+{
+    constexpr int KB = WideEnc<FORM>::kBytes;
+    constexpr int TPR = NT / TM;
+    constexpr int PPT = 16 / TPR;
+    static_assert(TPR * TM == NT && (TPR == 2 || TPR == 4), "2 or 4 threads per A row");
+    __shared__ __attribute__((aligned(16))) unsigned char s_tile[TM * KB];
+    const int kc = blockIdx.x, mtile = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int gr = tid / TPR, gh = tid % TPR;
+    const ARow ar = make_arow<FORM, PPT>(a, mtile * TM + gr, gh);
+    float4 xw[PPT / 2], cxy[PPT / 2];
+    const float4 *xp = reinterpret_cast<const float4 *>(ar.xw + kc * 16);
+    const float4 *cp = reinterpret_cast<const float4 *>(a.coords + PPT * gh + kc * 16);
+#pragma unroll
+    for (int u = 0; u < PPT / 2; ++u) {
+        xw[u] = xp[u];
+        cxy[u] = cp[u];
+    }
+    a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &s_tile[gr * KB]);
+    __syncthreads();
+    uint4 *dst = reinterpret_cast<uint4 *>(out + ((size_t)mtile * a.a_chunks + kc) * (TM * KB));
+    for (int u = tid; u < TM * KB / 16; u += NT) dst[u] = reinterpret_cast<const uint4 *>(s_tile)[u];
+}
+
+hipError_t launch_encode_a(int form, int tile_m, const CrossArgs &a, unsigned char *a_enc,
+                           hipStream_t s)
+{
+    const dim3 grid((unsigned)a.a_chunks, (unsigned)a.n_mtiles);
+    if (tile_m != 128 && tile_m != 256) return hipErrorInvalidValue;
+#define RC_ENC(F, TM) hipLaunchKernelGGL((encode_a_kernel<F, TM, 512>), grid, dim3(512), 0, s, a, a_enc)
+    switch (form) {
+    case kFP32: if (tile_m == 128) RC_ENC(kFP32, 128); else RC_ENC(kFP32, 256); break;
+    case kBF16X3: if (tile_m == 128) RC_ENC(kBF16X3, 128); else RC_ENC(kBF16X3, 256); break;
+    default: if (tile_m == 128) RC_ENC(kFP16, 128); else RC_ENC(kFP16, 256); break;
+    }
+#undef RC_ENC
+    return hipGetLastError();
+}
+
 // Operand registers of one 16-row (or 16-column) slice for one chunk.
 template <int FORM>
 struct WideFrag {
@@ -407,6 +470,8 @@ struct WideCfg {
 // STAGGER 1/2: the waves with bit 2 (1) or bit 0 (2) of their index set run
 // [build A of chunk k+1, MFMAs of chunk k] instead, so that the two waves a
 // SIMD holds keep its matrix core and its vector ALU busy at the same time.
+// STAGGER 8: the translation operand is precomputed (encode_a_kernel) and
+// copied global -> LDS like B, one chunk ahead.
 // STAGGER 4: each column slice's MFMAs share a fenced scheduling segment
 // with one pixel of the next chunk's A build.
 // epilogue output forms
@@ -456,16 +521,7 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
     // translation arrays (particle row_part[R]), so no particle is padded to a
     // multiple of 16 rows.  Rows past the last one are built from particle 0
     // with zero shift and never stored.
-    ARow ar;
-    {
-        const int R = mtile * TM + gr;
-        const bool ok = R < a.n_rows;
-        const int part = ok ? a.row_part[R] : 0;
-        ar.txr = ok ? (double)a.trans_x[R] * kInv2PiW : 0.0;
-        ar.tyr = ok ? (double)a.trans_y[R] * kInv2PiW : 0.0;
-        ar.scale = (FORM == kFP16) ? a.ascale[part] : 1.f;
-        ar.xw = a.xw + (size_t)part * a.kpad + PPT * gh;
-    }
+    const ARow ar = make_arow<FORM, PPT>(a, mtile * TM + gr, gh);
     const float2 *c_row = a.coords + PPT * gh;
 
     floatx4 acc[FN][FM];
@@ -530,28 +586,34 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
 
     // chunks past the projected pixels hold zeros: skip them
     const int nrun = (a.npix + 15) / 16;
-    const bool lead = STAGGER == 0 || STAGGER == 4 ||
-                      (((STAGGER == 1 ? (wave >> 2) : wave) & 1) == 0);
-    float4 xw[PPT / 2], cxy[PPT / 2];
-    issue_B(0, 0);
-    load_src(0, xw, cxy);
-    a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[0][gr * KB]);
-    if (!lead || STAGGER == 4) load_src(min(1, nrun - 1), xw, cxy);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    __syncthreads();
-
-    for (int kc = 0; kc < nrun; ++kc) {
-        const int buf = kc & 1;
-        // the last step fetches its own chunk again into the idle buffers,
-        // which keeps each path through the loop body one straight block
-        const int kn = min(kc + 1, nrun - 1);
-        if (STAGGER != 4) issue_B(kn, buf ^ 1);
-        if (STAGGER == 4) {
-            // one column slice of MFMAs and one pixel of the next chunk's A
-            // per fenced segment; the image data for the next build were
-            // loaded at the top of the previous step.  The fragment reads go
-            // first, so that the LDS serves them before the next chunk's
-            // copies land.
+    if constexpr (STAGGER == 8) {
+        // Translation operand precomputed for the whole launch: each chunk
+        // of this row tile is one contiguous block holding the exact LDS
+        // image of sA, copied like B (LPA 16-byte copies per thread).
+        constexpr int LPA = Cfg::kBufA / (NT * 16);
+        static_assert(LPA * NT * 16 == Cfg::kBufA, "A tile must split evenly into 16-byte loads");
+        const unsigned sA_lds = uniform32(
+            (unsigned)(size_t)(__attribute__((address_space(3))) void *)&sA[0][0] + wave * 64 * 16);
+        const unsigned long long a_base = uniform64(
+            (unsigned long long)(size_t)(a.a_enc + (size_t)mtile * a.a_chunks * Cfg::kBufA));
+        const uint32_t a_off = (uint32_t)(tid * 16);
+        auto issue_A = [&](int kc, int buf) {
+            const unsigned long long chunk =
+                a_base + (unsigned long long)(unsigned)kc * (unsigned)Cfg::kBufA;
+#pragma unroll
+            for (int it = 0; it < LPA; ++it)
+                lds_copy16(chunk + (unsigned long long)it * NT * 16, a_off,
+                           sA_lds + buf * Cfg::kBufA + it * NT * 16);
+        };
+        issue_B(0, 0);
+        issue_A(0, 0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        for (int kc = 0; kc < nrun; ++kc) {
+            const int buf = kc & 1;
+            const int kn = min(kc + 1, nrun - 1);
+            // fragment reads first, so that the LDS serves them before the
+            // next chunk's copies land
             const unsigned char *pa = sA[buf];
             const unsigned char *pb = sB[buf];
             WideFrag<FORM> fa_reg[FM];
@@ -562,28 +624,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
             }
             WideFrag<FORM> fb_reg[2];
             wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
-            // the copies below are invisible to the compiler's wait counting:
-            // have it wait for this step's image registers (loaded in the last
-            // step) before they are issued, not after
-#pragma unroll
-            for (int u = 0; u < PPT / 2; ++u)
-                asm volatile("" : "+v"(xw[u].x), "+v"(xw[u].y), "+v"(xw[u].z), "+v"(xw[u].w),
-                             "+v"(cxy[u].x), "+v"(cxy[u].y), "+v"(cxy[u].z), "+v"(cxy[u].w));
             issue_B(kn, buf ^ 1);
-            // 4-pixel threads load the image data of the step after this one
-            // a whole step early, into a second register set (8-pixel
-            // threads have no registers to spare: after the segments)
-            float4 xw2[PPT / 2], cxy2[PPT / 2];
-            if constexpr (PPT == 4) load_src(min(kn + 1, nrun - 1), xw2, cxy2);
-            float2 v[PPT];
-            uint32_t hw[PPT], lw[PPT];   // BF16x3, 4 pixels: each encoded in its own segment
-            // pixel j goes to segment j * FN / PPT + kPixOff: the segments
-            // that wait for the first fragments carry no pixel work
-            constexpr int kPixOff = FN > PPT ? FN / PPT - 1 : 0;
-            // raised issue priority while this wave has MFMAs to issue: the
-            // SIMD's other wave, past its MFMAs, yields to it (2-3 % here)
+            issue_A(kn, buf ^ 1);
             __builtin_amdgcn_s_setprio(1);
-            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int fb = 0; fb < FN; ++fb) {
                 if (fb + 1 < FN) {
@@ -591,41 +634,111 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel_wide(const CrossArg
                     wide_load<FORM>(fb_reg[(fb + 1) & 1], pb + rb * KB, rb, q);
                 }
                 wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg[fb & 1]);
-#pragma unroll
-                for (int j = 0; j < PPT; ++j)
-                    if ((j * FN) / PPT + kPixOff == fb) {
-                        v[j] = a_pixel<PPT>(ar, xw, cxy, j);
-                        if constexpr (FORM == kBF16X3 && PPT == 4)
-                            encode_bf16x3(v[j], hw[j], lw[j]);
-                    }
-                __builtin_amdgcn_sched_barrier(0);
             }
             __builtin_amdgcn_s_setprio(0);
-            if constexpr (FORM == kBF16X3 && PPT == 4)
-                a_store_words<PPT>(hw, lw, gh, gr, &sA[buf ^ 1][gr * KB]);
-            else
-                a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
-            if constexpr (PPT == 4) {
-#pragma unroll
-                for (int u = 0; u < PPT / 2; ++u) {
-                    xw[u] = xw2[u];
-                    cxy[u] = cxy2[u];
-                }
-            } else {
-                load_src(min(kn + 1, nrun - 1), xw, cxy);
-            }
-        } else if (lead) {
-            load_src(kn, xw, cxy);
-            mfma_chunk(buf);
-            a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
-        } else {
-            a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
-            load_src(min(kn + 1, nrun - 1), xw, cxy);
-            mfma_chunk(buf);
+            // this step's copies land before the barrier
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
         }
-        // the next chunk's B copies (and the image loads) land before the barrier
+    } else {
+        const bool lead = STAGGER == 0 || STAGGER == 4 ||
+                          (((STAGGER == 1 ? (wave >> 2) : wave) & 1) == 0);
+        float4 xw[PPT / 2], cxy[PPT / 2];
+        issue_B(0, 0);
+        load_src(0, xw, cxy);
+        a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[0][gr * KB]);
+        if (!lead || STAGGER == 4) load_src(min(1, nrun - 1), xw, cxy);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
+
+        for (int kc = 0; kc < nrun; ++kc) {
+            const int buf = kc & 1;
+            // the last step fetches its own chunk again into the idle buffers,
+            // which keeps each path through the loop body one straight block
+            const int kn = min(kc + 1, nrun - 1);
+            if (STAGGER != 4) issue_B(kn, buf ^ 1);
+            if (STAGGER == 4) {
+                // one column slice of MFMAs and one pixel of the next chunk's A
+                // per fenced segment; the image data for the next build were
+                // loaded at the top of the previous step.  The fragment reads go
+                // first, so that the LDS serves them before the next chunk's
+                // copies land.
+                const unsigned char *pa = sA[buf];
+                const unsigned char *pb = sB[buf];
+                WideFrag<FORM> fa_reg[FM];
+    #pragma unroll
+                for (int fa = 0; fa < FM; ++fa) {
+                    const int ra = ra0 + fa * 16 + r16;
+                    wide_load<FORM>(fa_reg[fa], pa + ra * KB, ra, q);
+                }
+                WideFrag<FORM> fb_reg[2];
+                wide_load<FORM>(fb_reg[0], pb + (rb0 + r16) * KB, rb0 + r16, q);
+                // the copies below are invisible to the compiler's wait counting:
+                // have it wait for this step's image registers (loaded in the last
+                // step) before they are issued, not after
+    #pragma unroll
+                for (int u = 0; u < PPT / 2; ++u)
+                    asm volatile("" : "+v"(xw[u].x), "+v"(xw[u].y), "+v"(xw[u].z), "+v"(xw[u].w),
+                                 "+v"(cxy[u].x), "+v"(cxy[u].y), "+v"(cxy[u].z), "+v"(cxy[u].w));
+                issue_B(kn, buf ^ 1);
+                // 4-pixel threads load the image data of the step after this one
+                // a whole step early, into a second register set (8-pixel
+                // threads have no registers to spare: after the segments)
+                float4 xw2[PPT / 2], cxy2[PPT / 2];
+                if constexpr (PPT == 4) load_src(min(kn + 1, nrun - 1), xw2, cxy2);
+                float2 v[PPT];
+                uint32_t hw[PPT], lw[PPT];   // BF16x3, 4 pixels: each encoded in its own segment
+                // pixel j goes to segment j * FN / PPT + kPixOff: the segments
+                // that wait for the first fragments carry no pixel work
+                constexpr int kPixOff = FN > PPT ? FN / PPT - 1 : 0;
+                // raised issue priority while this wave has MFMAs to issue: the
+                // SIMD's other wave, past its MFMAs, yields to it (2-3 % here)
+                __builtin_amdgcn_s_setprio(1);
+                __builtin_amdgcn_sched_barrier(0);
+    #pragma unroll
+                for (int fb = 0; fb < FN; ++fb) {
+                    if (fb + 1 < FN) {
+                        const int rb = rb0 + (fb + 1) * 16 + r16;
+                        wide_load<FORM>(fb_reg[(fb + 1) & 1], pb + rb * KB, rb, q);
+                    }
+                    wide_mfma<FORM, FM>(acc[fb], fa_reg, fb_reg[fb & 1]);
+    #pragma unroll
+                    for (int j = 0; j < PPT; ++j)
+                        if ((j * FN) / PPT + kPixOff == fb) {
+                            v[j] = a_pixel<PPT>(ar, xw, cxy, j);
+                            if constexpr (FORM == kBF16X3 && PPT == 4)
+                                encode_bf16x3(v[j], hw[j], lw[j]);
+                        }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                __builtin_amdgcn_s_setprio(0);
+                if constexpr (FORM == kBF16X3 && PPT == 4)
+                    a_store_words<PPT>(hw, lw, gh, gr, &sA[buf ^ 1][gr * KB]);
+                else
+                    a_store<FORM, PPT>(ar, v, gh, gr, &sA[buf ^ 1][gr * KB]);
+                if constexpr (PPT == 4) {
+    #pragma unroll
+                    for (int u = 0; u < PPT / 2; ++u) {
+                        xw[u] = xw2[u];
+                        cxy[u] = cxy2[u];
+                    }
+                } else {
+                    load_src(min(kn + 1, nrun - 1), xw, cxy);
+                }
+            } else if (lead) {
+                load_src(kn, xw, cxy);
+                mfma_chunk(buf);
+                a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+            } else {
+                a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[buf ^ 1][gr * KB]);
+                load_src(min(kn + 1, nrun - 1), xw, cxy);
+                mfma_chunk(buf);
+            }
+            // the next chunk's B copies (and the image loads) land before the barrier
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
+
     }
 
     // ---- epilogue: float64 assembly, stores, per-row (min, argmin) partials.
@@ -887,6 +1000,8 @@ static VariantEntry wide_one()
         wide_one<F, 2, 4, 4, 8, 0>(), /* 128 x 512, 512 threads */                     \
         wide_one<F, 2, 4, 4, 8, 1>(), /* 128 x 512, staggered by wave bit 2 */         \
         wide_one<F, 2, 4, 4, 8, 4>(), /* 128 x 512, fenced segments */                 \
+        wide_one<F, 2, 4, 4, 8, 8>(), /* 128 x 512, translation operand precomputed */ \
+        wide_one<F, 4, 2, 4, 8, 8>(), /* 256 x 256, translation operand precomputed */ \
     };
 
 RC_WIDE_TABLE(kFP32)

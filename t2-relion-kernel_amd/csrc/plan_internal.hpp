@@ -31,6 +31,8 @@ struct rc_plan {
     float *pabs2, *pnorm;
     double *part_min;
     long long *part_idx;
+    unsigned char *a_enc;                   // schedule-8 variants: precomputed A
+    int a_chunks;                           // 16-pixel chunks per row tile of a_enc
     std::vector<hipEvent_t> ev;             // timing slots: 2 per launch
     long long launches = 0;
     bool capturing = false;

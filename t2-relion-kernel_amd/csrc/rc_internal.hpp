@@ -99,6 +99,11 @@ struct CrossArgs {
     int ksplit, kc_per_split;
     float *part_c;               // [ksplit][O][n_trans] cross-term partials
     float *part_pn;              // [ksplit][O] sum corr/2 |P|^2 partials
+    // wide kernels with the translation operand precomputed (schedule 8):
+    // [row tile][chunk][tile_m rows][row bytes], each chunk block the exact
+    // (swizzled) LDS image of the tile's A buffer; a_chunks = chunks per row tile
+    const unsigned char *a_enc;
+    int a_chunks;
     // outputs
     const float *diff2_init;     // same layout as diff2, may be NULL, may alias diff2
     float *diff2;
@@ -117,7 +122,8 @@ struct VariantInfo {
     bool persistent;
     bool wide = false;     // 16-pixel chunks, wide projection encoding
     const char *family = "cross_kernel";   // kernel template name
-    int schedule = 0;      // wide kernels: wave schedule (0 plain, 1/2 staggered, 4 fenced)
+    int schedule = 0;      // wide kernels: wave schedule (0 plain, 1/2 staggered, 4 fenced,
+                           // 8 translation operand precomputed and copied like B)
 };
 
 struct VariantEntry {
@@ -142,6 +148,12 @@ hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const
 hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,
                           const float *eulers, int O, const float2 *coords, int npix, int kpad,
                           unsigned char *proj, float *pabs2, hipStream_t s);
+
+// Translation operand of the wide kernels with precomputed A (schedule 8):
+// a.a_enc[row tile][chunk] = the LDS image of the tile's A chunk, for rows of
+// tile_m (128 or 256) built by the same arithmetic as the in-kernel build.
+hipError_t launch_encode_a(int form, int tile_m, const CrossArgs &a, unsigned char *a_enc,
+                           hipStream_t s);
 
 // Projections in the wide kernels' encoding: [C][kpad/16][O][16-pixel chunk].
 hipError_t launch_project_wide(int form, const ProjView *vols, const float *bscale, int n_classes,

@@ -213,6 +213,16 @@ def make_stream(n_requests, rate_hz, seed=7, particles=(20, 400), classes=(1, 4)
     return out, cache
 
 
+def resample_stream(base, n_requests, rate_hz, seed=0):
+    """n_requests Poisson arrivals at rate_hz, each request drawn uniformly
+    from the generated stream `base` (its host inputs are submitted again in
+    full, so the server does the whole work each time)."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_hz, n_requests))
+    pick = rng.integers(0, len(base), n_requests)
+    return [(float(t[k]), base[int(pick[k])][1]) for k in range(n_requests)]
+
+
 def run_stream(server, stream, time_scale=1.0):
     """Submit each request at its arrival time (from a submitter thread),
     wait for all, return per-request latencies (ms) and the wall time."""

@@ -426,3 +426,47 @@ def _deterministic_alone_and_in_batch(c1, lib, torch_cuda):
         plan.execute(t1)
         torch.cuda.synchronize()
         assert np.array_equal(t1["diff2"].cpu().numpy(), d_full[off[i]:off[i + 1]])
+
+
+def _plain_and_precomputed(lib, form):
+    """(plain-schedule variant, translation-operand-precomputed variant) pairs
+    of the same 512-thread wide tile shape."""
+    names = {v: lib.variant_name(form, v) for v, _, _ in lib.variants(form)}
+    pairs = []
+    for v, n in names.items():
+        if "translation operand precomputed" not in n:
+            continue
+        shape = n.split(",")[0]                 # "cross_kernel_wide<...> 128x512"
+        plain = [u for u, m in names.items()
+                 if m.startswith(shape + ", 512 threads") and m.count(",") == 2]
+        assert len(plain) == 1, (n, plain)
+        pairs.append((plain[0], v))
+    return pairs
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
+def test_precomputed_operand_is_bit_identical(c1, lib, torch_cuda, form):
+    """The precomputed translation operand is built by the same arithmetic
+    as the in-kernel build and the MFMAs run in the same order, so each
+    such variant reproduces the plain variant of its tile shape bit for bit
+    (the 32^2 workload and the ragged two-class batch)."""
+    torch = torch_cuda
+    pairs = _plain_and_precomputed(lib, form)
+    assert len(pairs) >= 2
+    for prob in (c1[0], _ragged_problem()):
+        projs = [lib.Projector(v) for v in prob.vols]
+        try:
+            for plain, pre in pairs:
+                got = []
+                for v in (plain, pre):
+                    lib.set_variant_override(v)
+                    plan = lib.plan_for_problem(prob, projs, form)
+                    t = lib.problem_tensors(prob)
+                    plan.execute(t)
+                    torch.cuda.synchronize()
+                    got.append((t["diff2"].cpu().numpy(), t["best_index"].cpu().numpy()))
+                    plan.close()
+                assert np.array_equal(got[0][0], got[1][0]), (form, plain, pre)
+                assert np.array_equal(got[0][1], got[1][1]), (form, plain, pre)
+        finally:
+            lib.set_variant_override(-1)
