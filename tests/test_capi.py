@@ -93,3 +93,34 @@ def test_single_run_validates_before_launch():
     st = L.rc_coarse_run(21, 576, 414, 544, 0, None, None, None, None, None, C.byref(k), None,
                          None, None, None, None, None, 0)
     assert st == 1
+
+
+def test_new_entry_points_validate_before_launch():
+    """The split query, the partition calls and the allocation-free drop-in
+    reject bad arguments with status codes, touching no device."""
+    L = lib.load()
+    ks, kc = C.c_int(), C.c_int()
+    assert L.rc_coarse_split(0, 4608, 1636, 0, C.byref(ks), C.byref(kc)) == 1
+    assert L.rc_coarse_split(113, 4608, 1636, 9, C.byref(ks), C.byref(kc)) == 1
+    assert L.rc_coarse_split(113, 4608, 1636, 1, C.byref(ks), C.byref(kc)) == 0
+    assert ks.value > 1 and (ks.value - 1) * kc.value < 52 <= ks.value * kc.value
+    # one slice when the image alone fills the device or has one chunk
+    assert L.rc_coarse_split(113, 4608 * 64, 1636, 1, C.byref(ks), C.byref(kc)) == 0
+    assert ks.value == 1
+    assert L.rc_coarse_split(21, 576, 20, 1, C.byref(ks), C.byref(kc)) == 0
+    assert ks.value == 1 and kc.value == 1
+    view = lib.geometry_view(17, 32, 16, 2.0)
+    n = C.c_int()
+    assert L.rc_filter_coarse_order(None, 544, None, C.byref(n)) == 1
+    assert L.rc_filter_coarse_order(C.byref(view), 0, None, C.byref(n)) == 1
+    assert L.rc_filter_coarse_images(C.byref(view), 544, 1, None, None, None, None, None, None,
+                                     None, None) == 1
+    assert L.rc_filter_coarse_images(C.byref(view), 545 + 17 * 32, 1, None, None, None, None,
+                                     None, None, None, None) == 1
+    k = lib.rc_projector_kernel()
+    st = L.rc_run_diff2_kernel_coarse_noalloc(C.byref(k), None, None, None, None, None, None,
+                                              None, None, 1.0, 1, 1, 1, None, 0, 1)
+    assert st == 2 and b"data_is_3D" in L.rc_last_error()
+    st = L.rc_run_diff2_kernel_coarse_noalloc(C.byref(k), None, None, None, None, None, None,
+                                              None, None, 1.0, 1, 1, 1, None, 0, 0)
+    assert st == 1

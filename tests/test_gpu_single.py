@@ -181,6 +181,40 @@ def test_filter_images_on_device(lib, torch_cuda):
     assert np.all(out["filtered_size"].cpu().numpy() == n_in)
 
 
+def test_filtered_images_feed_a_batch_plan(lib, torch_cuda):
+    """The C-only preparation path: unfiltered images partitioned on the
+    device by rc_filter_coarse_images and handed straight to a batch plan
+    give the same bits as the host-partitioned inputs, and match float64."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob = synth.make_problem("c1", seed=11)
+    pb, g = prob.particles, prob.geom
+    n = g.image_size
+    inv = _inverse(synth.filter_order(g)[0])
+    fimg = np.concatenate([pb.real[:, inv], pb.imag[:, inv]], axis=1)
+    out = lib.filter_images(lib.geometry_view(g.imgX, g.imgY, g.maxR, g.pad), n,
+                            _t(torch, fimg), _t(torch, pb.corr[:, inv]))
+    assert int(out["filtered_size"][0]) == pb.npix
+    projs = [lib.Projector(v) for v in prob.vols]
+    host = lib.problem_tensors(prob)
+    dev = dict(host, real=out["real"].reshape(-1), imag=out["imag"].reshape(-1),
+               corr=out["corr"].reshape(-1), coor_xy=out["coor_xy"],
+               diff2=torch.empty_like(host["diff2"]),
+               best_diff2=torch.empty_like(host["best_diff2"]),
+               best_index=torch.empty_like(host["best_index"]))
+    plan = lib.plan_for_problem(prob, projs, "bf16x3")
+    plan.execute(host)
+    plan.execute(dev)
+    torch.cuda.synchronize()
+    assert torch.equal(host["diff2"], dev["diff2"])
+    assert torch.equal(host["best_index"], dev["best_index"])
+    off = lib.output_offsets(prob.trans_offset, prob.n_classes, prob.orientation_num)
+    ref = oracle.problem_diff2_expanded(prob, [0])
+    got = dev["diff2"][off[0]:off[1]].cpu().numpy().astype(np.float64)
+    _check(got, ref[0].ravel(), "bf16x3", "filtered c1 particle 0")
+    plan.close()
+
+
 def test_autotune_choice_reaches_later_plans(lib, torch_cuda):
     """rc_autotune returns one of the listed variants, with a time, and a
     plan created afterwards for the same shape uses it; the process-wide
