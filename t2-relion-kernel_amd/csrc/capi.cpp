@@ -384,26 +384,31 @@ namespace rc {
 
 // Variant used when the autotuner has not seen a shape.  Measured on MI355X
 // at 512 particles x 113 translations x 4608 orientations x 1636 pixels:
-// BF16X3 and FP16 run fastest on the wide 128 x 512 kernel with fenced
-// segments (translation operand built in the kernel, 4.0-4.1 ms); FP32 on
-// the wide 256 x 256 kernel with the translation operand precomputed (12.4
-// vs 13.2 ms: its longer MFMA phase hides the operand copies, while the
-// VALU build competes with its FP32 MFMAs).  These need enough tiles to fill
-// the device and, for 512 columns, little column waste; then the wide
-// 256 x 256 kernel with staggered waves; smaller searches use the pipelined
-// 128 x 128 kernel (FP16: 64 x 256), whose finer tiles waste less padding.
-// rc_autotune overrides this per shape.
-static int default_variant(int form, int n_rows, int n_classes, int O)
+// BF16X3 and FP16 run fastest on the direct 128 x 512 kernel (projections
+// loaded into registers; BF16X3 3.8-3.9 ms, 2-4 % ahead of the wide
+// 128 x 512 kernels; FP16 2.0 ms, 12 % ahead), which takes batches whose
+// 128-row tiles hold at most kDirectMaxTileParts particles, else the wide
+// 128 x 512 kernel with fenced segments; FP32 on the wide 256 x 256 kernel
+// with the translation operand precomputed (12.4 vs 13.2 ms: its longer
+// MFMA phase hides the operand copies, while the VALU build competes with
+// its FP32 MFMAs).  These need enough tiles to fill the device and, for
+// 512 columns, little column waste; then the wide 256 x 256 kernel with
+// staggered waves; smaller searches use the pipelined 128 x 128 kernel
+// (FP16: 64 x 256), whose finer tiles waste less padding.  rc_autotune
+// overrides this per shape.
+static int default_variant(int form, int n_rows, int n_classes, int O, int tile128_parts)
 {
     constexpr int kWide256 = 14;     // wide, 256 x 256, staggered waves
     constexpr int kWide512 = 21;     // wide, 128 x 512, fenced segments
     constexpr int kWide256Pre = 23;  // wide, 256 x 256, translation operand precomputed
+    constexpr int kDirect = 24;      // direct, 128 x 512
     const long long rows = n_rows;   // the wide kernels pack the rows
     const long long t512 = ((rows + 127) / 128) * n_classes * ((O + 511) / 512);
     const long long t256 = ((rows + 255) / 256) * n_classes * ((O + 255) / 256);
     const bool narrow_waste = (long long)((O + 511) / 512) * 512 > O + O / 8;
     if (form == kFP32 && t256 >= 1024) return kWide256Pre;
-    if (t512 >= 1024 && !narrow_waste) return kWide512;
+    if (t512 >= 1024 && !narrow_waste)
+        return tile128_parts <= kDirectMaxTileParts ? kDirect : kWide512;
     if (t256 >= 512) return kWide256;
     return form == kFP16 ? 7 : 4;
 }
@@ -451,8 +456,11 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     p->ascale = c.take<float>(p->P);
     // padding: the wide kernels' copies of a last, partial column tile run
     // up to one tile (512 columns x 128 bytes) past the last projection
-    p->proj = c.take<unsigned char>((size_t)p->C * p->O * p->nkc * chunk_bytes(p->form) +
-                                    kProjPad);
+    p->proj = c.take<unsigned char>(
+        (p->info.wide ? projection_bytes(p->form, p->info.schedule == kScheduleDirect, p->C,
+                                         p->O, p->kpad)
+                      : (size_t)p->C * p->O * p->nkc * chunk_bytes(p->form)) +
+        kProjPad);
     p->pabs2 = c.take<float>((size_t)p->C * p->O * p->kpad);
     p->pnorm = c.take<float>((size_t)p->P * p->C * p->O);
     const size_t prows = p->packed ? (size_t)p->n_rows : (size_t)p->n_groups;
@@ -471,6 +479,16 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
 }  // namespace rc
 
 namespace rc {
+
+// Largest number of particles among the row tiles of `tile_m` packed rows.
+static int max_tile_particles(const std::vector<int> &row_part, int tile_m)
+{
+    int worst = 0;
+    const int n = (int)row_part.size();
+    for (int r0 = 0; r0 < n; r0 += tile_m)
+        worst = std::max(worst, row_part[std::min(r0 + tile_m, n) - 1] - row_part[r0] + 1);
+    return worst;
+}
 
 int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab, int forced_variant)
 {
@@ -501,11 +519,22 @@ int plan_setup(rc_plan *p, const rc_batch_desc *d, PlanTables &tab, int forced_v
     if (vi < 0) {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tuned.find(tune_key(d));
-        vi = it != g_tuned.end() ? it->second
-                                  : default_variant(p->form, p->trans_offset[p->P], p->C, p->O);
+        vi = it != g_tuned.end()
+                 ? it->second
+                 : default_variant(p->form, p->trans_offset[p->P], p->C, p->O,
+                                   max_tile_particles(tab.row_part, 128));
     }
     const VariantInfo *info = variant_info(p->form, false, vi);
     RC_REQUIRE(info, RC_ERR_INVALID, "variant %d does not exist for form %d", vi, d->form);
+    if (info->max_tile_parts > 0) {
+        // rows are packed in tiles of tile_m; the kernel stages the image
+        // data of every particle of a tile
+        const int worst = max_tile_particles(tab.row_part, info->tile_m);
+        RC_REQUIRE(worst <= info->max_tile_parts, RC_ERR_UNSUPPORTED,
+                   "variant %d takes at most %d particles per %d-row tile; this batch puts %d "
+                   "in one",
+                   vi, info->max_tile_parts, info->tile_m, worst);
+    }
     p->variant = vi;
     p->info = *info;
     p->packed = info->wide;
@@ -584,7 +613,8 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
                        p->half_corr, p->ascale, s, d->do_cc != 0));
     if (p->info.wide)
         RC_HIP(launch_project_wide(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O,
-                                   p->coords, p->npix, p->kpad, p->proj, p->pabs2, s));
+                                   p->coords, p->npix, p->kpad, p->proj, p->pabs2,
+                                   p->info.schedule == kScheduleDirect, s));
     else
         RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
                               p->npix, p->kpad, p->proj, p->pabs2, s));
@@ -902,19 +932,27 @@ int rc_autotune(const rc_batch_desc *d, void *stream, int iters, int *best_varia
     // Each plan is made with its variant named explicitly, so plans created
     // meanwhile by other threads are not affected by the sweep.
     // Screen: one untimed and one timed launch per variant.
-    std::vector<float> first(n);
-    float fastest = 0.f;
+    // Variants that cannot take this batch (RC_ERR_UNSUPPORTED from plan
+    // creation) are left out.
+    std::vector<float> first(n, -1.f);
+    float fastest = -1.f;
     for (int v = 0; v < n; ++v) {
-        if (int st = time_variant(d, s, v, 1, 1, &first[v])) return st;
-        if (v == 0 || first[v] < fastest) fastest = first[v];
+        const int st = time_variant(d, s, v, 1, 1, &first[v]);
+        if (st == RC_ERR_UNSUPPORTED) {
+            first[v] = -1.f;
+            continue;
+        }
+        if (st) return st;
+        if (fastest < 0.f || first[v] < fastest) fastest = first[v];
     }
+    RC_REQUIRE(fastest >= 0.f, RC_ERR_UNSUPPORTED, "no variant takes this batch");
     // Refine the variants within 25 % of the fastest screen: two sweeps of
     // two untimed and `iters` timed launches, each keeping its better
     // median (a variant timed early is not penalised by a clock still
     // ramping up).
     std::vector<int> cand;
     for (int v = 0; v < n; ++v)
-        if (first[v] <= 1.25f * fastest) cand.push_back(v);
+        if (first[v] >= 0.f && first[v] <= 1.25f * fastest) cand.push_back(v);
     std::vector<float> vt(n, -1.f);
     for (int sweep = 0; sweep < 2; ++sweep)
         for (int v : cand) {
@@ -1058,6 +1096,7 @@ const char *rc_variant_name(int form, int variant)
              : v->schedule == 2 ? ", staggered waves (bit 0)"
              : v->schedule == 4 ? ", fenced segments"
              : v->schedule == 8 ? ", translation operand precomputed"
+             : v->schedule == kScheduleDirect ? ", projections loaded into registers"
                                 : "");
     return buf;
 }

@@ -869,17 +869,22 @@ static const VariantEntry *entry(int form, bool fused, int idx)
     if (!t || idx < 0) return nullptr;
     if (idx < n) return &t[idx];
     if (fused) return nullptr;
-    int nw = 0;
+    int nw = 0, nd = 0;
     const VariantEntry *w = wide_table(form, &nw);
-    return idx - n < nw ? &w[idx - n] : nullptr;
+    if (idx - n < nw) return &w[idx - n];
+    const VariantEntry *d = direct_table(form, &nd);
+    return idx - n - nw < nd ? &d[idx - n - nw] : nullptr;
 }
 
 int variant_count(int form, bool fused)
 {
-    int n = 0, nw = 0;
+    int n = 0, nw = 0, nd = 0;
     table(form, fused, &n);
-    if (!fused && n) wide_table(form, &nw);
-    return n + nw;
+    if (!fused && n) {
+        wide_table(form, &nw);
+        direct_table(form, &nd);
+    }
+    return n + nw + nd;
 }
 
 const VariantInfo *variant_info(int form, bool fused, int idx)

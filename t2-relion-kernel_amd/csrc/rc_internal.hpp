@@ -123,8 +123,17 @@ struct VariantInfo {
     bool wide = false;     // 16-pixel chunks, wide projection encoding
     const char *family = "cross_kernel";   // kernel template name
     int schedule = 0;      // wide kernels: wave schedule (0 plain, 1/2 staggered, 4 fenced,
-                           // 8 translation operand precomputed and copied like B)
+                           // 8 translation operand precomputed and copied like B,
+                           // kScheduleDirect projections loaded straight into registers)
+    int max_tile_parts = 0;  // > 0: at most this many particles in one row tile
 };
+
+// Schedule of the direct kernels (kernels_direct.hip), whose projections use
+// the direct layout (launch_project_wide, direct = true).
+constexpr int kScheduleDirect = 16;
+// Particles one row tile of the direct kernels may hold (their image data
+// is staged in LDS per tile).
+constexpr int kDirectMaxTileParts = 31;
 
 struct VariantEntry {
     VariantInfo info;
@@ -133,6 +142,8 @@ struct VariantEntry {
 
 // Wide kernels (both operands staged in LDS, 16-pixel chunks).
 const VariantEntry *wide_table(int form, int *n);
+// Direct kernels (projections loaded into registers, 16-pixel chunks).
+const VariantEntry *direct_table(int form, int *n);
 
 int variant_count(int form, bool fused);
 const VariantInfo *variant_info(int form, bool fused, int idx);
@@ -155,10 +166,16 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
 hipError_t launch_encode_a(int form, int tile_m, const CrossArgs &a, unsigned char *a_enc,
                            hipStream_t s);
 
-// Projections in the wide kernels' encoding: [C][kpad/16][O][16-pixel chunk].
+// Projections in the wide kernels' encoding: [C][kpad/16][O][16-pixel chunk]
+// or, direct = true, [C][kpad/16][ceil(O/16)][16-orientation group], each
+// group block holding the group's MFMA fragments in lane order
+// (direct_seg_offset; orientations past O are zero).
 hipError_t launch_project_wide(int form, const ProjView *vols, const float *bscale, int n_classes,
                                const float *eulers, int O, const float2 *coords, int npix,
-                               int kpad, unsigned char *proj, float *pabs2, hipStream_t s);
+                               int kpad, unsigned char *proj, float *pabs2, bool direct,
+                               hipStream_t s);
+// Bytes of the projection buffer of a plan (without the trailing kProjPad).
+size_t projection_bytes(int form, bool direct, int n_classes, int O, int kpad);
 
 // pnorm[i][c*O+o] = sum_p half_corr[i][p] * pabs2[c*O+o][p]
 hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half_corr,

@@ -175,10 +175,17 @@ def _wide_256(lib, form):
 
 def _wide_512(lib, form):
     """Indices of the 128 x 512 wide kernels (the large-batch default)."""
-    return [v for v, tm, tn in lib.variants(form) if tm == 128 and tn == 512]
+    return [v for v, tm, tn in lib.variants(form)
+            if tm == 128 and tn == 512 and lib.variant_name(form, v).startswith("cross_kernel_wide")]
 
 
-@pytest.mark.parametrize("kernel", ["default", "wide", "wide512"])
+def _direct(lib, form):
+    """Indices of the direct kernels (projections loaded into registers)."""
+    return [v for v, _, _ in lib.variants(form)
+            if lib.variant_name(form, v).startswith("cross_kernel_direct")]
+
+
+@pytest.mark.parametrize("kernel", ["default", "wide", "wide512", "direct"])
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
 def test_batched_cc(c1, lib, torch_cuda, form, kernel):
     """Batched cross-correlation scores; the per-particle minimum is the best
@@ -197,6 +204,8 @@ def test_batched_cc(c1, lib, torch_cuda, form, kernel):
             lib.set_variant_override(_wide_256(lib, form)[0])
         elif kernel == "wide512":
             lib.set_variant_override(_wide_512(lib, form)[-1])
+        elif kernel == "direct":
+            lib.set_variant_override(_direct(lib, form)[0])
         plan = lib.plan_for_problem(prob, projs, form, cc=True)
         t = lib.problem_tensors(prob)
         plan.execute(t)
@@ -468,5 +477,38 @@ def test_precomputed_operand_is_bit_identical(c1, lib, torch_cuda, form):
                     plan.close()
                 assert np.array_equal(got[0][0], got[1][0]), (form, plain, pre)
                 assert np.array_equal(got[0][1], got[1][1]), (form, plain, pre)
+        finally:
+            lib.set_variant_override(-1)
+
+
+@pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
+def test_direct_kernel_is_bit_identical(c1, lib, torch_cuda, form):
+    """The direct kernel (projections loaded straight into registers, one
+    wave per SIMD) builds the same operands and runs every accumulator's
+    MFMAs in the same order as the wide 128 x 512 kernels, so it reproduces
+    them bit for bit: the 32^2 workload (a partial last column tile of 64
+    orientations) and the ragged two-class batch (a partial last row tile)."""
+    torch = torch_cuda
+    names = {v: lib.variant_name(form, v) for v, _, _ in lib.variants(form)}
+    plain = [v for v, n in names.items()
+             if n.startswith("cross_kernel_wide") and "128x512, 512 threads" in n
+             and n.count(",") == 2]
+    direct = _direct(lib, form)
+    assert len(plain) == 1 and len(direct) == 1, names
+    for prob in (c1[0], _ragged_problem()):
+        projs = [lib.Projector(v) for v in prob.vols]
+        try:
+            got = []
+            for v in (plain[0], direct[0]):
+                lib.set_variant_override(v)
+                plan = lib.plan_for_problem(prob, projs, form)
+                t = lib.problem_tensors(prob)
+                plan.execute(t)
+                torch.cuda.synchronize()
+                got.append((t["diff2"].cpu().numpy(), t["best_index"].cpu().numpy(),
+                            t["best_diff2"].cpu().numpy()))
+                plan.close()
+            for a, b in zip(got[0], got[1]):
+                assert np.array_equal(a, b), (form, plain, direct)
         finally:
             lib.set_variant_override(-1)
