@@ -366,7 +366,9 @@ def downstream_bench(lib, torch, prob, t, steps, cpu_seconds):
         roofline=dict(bound="hbm", achieved=algo / (ms * 1e-3) / 1e9, peak=8000.0, unit="GB/s",
                       frac=algo / (ms * 1e-3) / 1e9 / 8000.0,
                       bytes_per_launch=algo, note="algorithmic bytes: 4 B read + 5 B written "
-                      "per sample; the kernel reads each weight once more per selection pass"),
+                      "per sample; the kernel moves these alone when at most 4096 weights of a "
+                      "particle are positive, else it reads the weights once more and writes "
+                      "the mask bytes twice (14 B per sample)"),
         significant_per_particle_median=float(np.median(nsig)),
         workload=f"{P} particles x {samples // P} samples, adaptive fraction 0.999")
     # ---- fine pass of particle 0

@@ -28,12 +28,24 @@
 // every run and do not depend on the other particles in the batch.  (A bound
 // of exactly 0, fraction 1, is crossed by the first positive weight.)
 //
-// Memory passes over the particle's samples: (1) read diff2 for the maximum;
-// (2) read diff2, write the weights, first-level histogram; (3) read the
-// weights: the samples outside the selected first-level bucket get their
-// mask byte, those inside (at most kWCap of them) are kept in LDS, where the
-// last two levels and their mask bytes are finished.  A bucket holding more
-// than kWCap samples takes two more histogram passes and a mask pass instead.
+// Memory passes over the particle's samples: (1) read diff2 for the maximum
+// -- skipped without priors, see below; (2) read diff2, write the weights and
+// a zero mask byte, first-level histogram, and every positive weight kept in
+// an LDS list.  The posterior is sharp, so most weights round to 0 and the
+// list (at most kWCap entries) usually holds all positive ones: the last two
+// radix levels and the mask bytes of the positive samples are then finished
+// from LDS, 9 bytes of memory traffic per sample (13 with priors).  With more
+// positive weights: (3) read the weights: the samples outside the selected
+// first-level bucket get their mask byte, those inside (at most kWCap) go to
+// LDS; a bucket holding more than kWCap samples takes two more histogram
+// passes and a mask pass instead.
+//
+// Without priors every counted sample has lw = min_i - diff2_k <= min_i -
+// minf (samples below minf, the float the search stores for the minimum, get
+// w = 0), with equality at the minimum's own sample: pass 2 takes that bound
+// as the maximum, checks it against the largest lw it sees, and is repeated
+// with the true maximum when no sample attains the bound (only when the
+// weights do not overwrite diff2; in place, pass 1 runs).
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -305,9 +317,21 @@ __global__ __launch_bounds__(kWT, 2) void weights_kernel(const WeightsArgs a)
         return (double)po + (double)pt + (mind - (double)dk);
     };
 
-    // ---- pass 1: maximum log weight
-    double lmax = -DBL_MAX;
-    {
+    // block-wide maximum of one value per thread
+    auto block_max = [&](double v) {
+        for (int o2 = 32; o2 > 0; o2 >>= 1) v = fmax(v, __shfl_xor(v, o2, 64));
+        if (lane == 0) s_red[wave] = v;
+        __syncthreads();
+        v = s_red[0];
+        for (int u = 1; u < kWT / 64; ++u) v = fmax(v, s_red[u]);
+        __syncthreads();
+        return v;
+    };
+
+    // ---- pass 1: maximum log weight (with priors, or in place)
+    const bool bound_max = !priors && (const void *)a.weights != (const void *)a.diff2;
+    double lmax = bound_max ? mind - (double)minf : -DBL_MAX;
+    if (!bound_max) {
         auto one = [&](long long k, float dk) {
             bool ok;
             const double lw = log_weight(k, dk, ok);
@@ -321,32 +345,45 @@ __global__ __launch_bounds__(kWT, 2) void weights_kernel(const WeightsArgs a)
                   one(k + 3, x.w);
               },
               one);
+        lmax = block_max(lmax);
     }
-    for (int o2 = 32; o2 > 0; o2 >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o2, 64));
-    if (lane == 0) s_red[wave] = lmax;
-    clear_hist();
-    __syncthreads();
-    lmax = s_red[0];
-    for (int u = 1; u < kWT / 64; ++u) lmax = fmax(lmax, s_red[u]);
-    const bool any = lmax > -DBL_MAX;
+    bool any = lmax > -DBL_MAX;
 
-    // ---- pass 2: weights and the first-level histogram
-    {
+    // ---- pass 2: weights, zero mask bytes, the first-level histogram and
+    // the list of positive weights; repeated once when the bound taken as
+    // the maximum is not attained
+    for (int attempt = 0;; ++attempt) {
+        clear_hist();
+        if (tid == 0) s_ncap = 0u;
+        __syncthreads();
+        double seen = -DBL_MAX;   // largest log weight of a counted sample
         // exp(x) rounds to float 0 for x < -104 (below 2^-150)
         auto weight = [&](long long k, float dk) {
             bool ok;
-            const double x = log_weight(k, dk, ok) - lmax;
+            const double lw = log_weight(k, dk, ok);
+            if (ok) seen = fmax(seen, lw);
+            const double x = lw - lmax;
             return (ok && any) ? (x < -104.0 ? 0.f : (float)exp(x)) : 0.f;
         };
-        auto count = [&](float wk) {
+        auto count = [&](long long k, float wk) {
             if (wk > 0.f) {
                 const unsigned bits = __float_as_uint(wk);
                 atomicAdd(&h_sum[bits >> wshift(0)], significand(bits));
                 atomicAdd(&h_cnt[bits >> wshift(0)], 1u);
+                // once the list has overflowed (dense weights) the shared
+                // counter is left alone: it only records the overflow
+                if (*(volatile unsigned *)&s_ncap <= (unsigned)kWCap) {
+                    const unsigned slot = atomicAdd(&s_ncap, 1u);
+                    if (slot < (unsigned)kWCap) {
+                        cap_val[slot] = wk;
+                        cap_idx[slot] = (int)k;
+                    }
+                }
             }
         };
         long long hw = (long long)(((16u - ((unsigned)(uintptr_t)d & 15u)) & 15u) >> 2);
         const bool wvec = (((uintptr_t)(w + hw)) & 15u) == 0;
+        const bool mvec = mk && (((uintptr_t)(mk + hw)) & 3u) == 0;
         sweep(d, n,
               [&](long long k, float4 x) {
                   const float4 r = make_float4(weight(k, x.x), weight(k + 1, x.y),
@@ -359,18 +396,28 @@ __global__ __launch_bounds__(kWT, 2) void weights_kernel(const WeightsArgs a)
                       w[k + 2] = r.z;
                       w[k + 3] = r.w;
                   }
-                  count(r.x);
-                  count(r.y);
-                  count(r.z);
-                  count(r.w);
+                  if (mk) store_mask4(mk, k, mvec, 0, 0, 0, 0);
+                  count(k, r.x);
+                  count(k + 1, r.y);
+                  count(k + 2, r.z);
+                  count(k + 3, r.w);
               },
               [&](long long k, float dk) {
                   const float r = weight(k, dk);
                   w[k] = r;
-                  count(r);
+                  if (mk) mk[k] = 0;
+                  count(k, r);
               });
+        // Each wave's stores of this pass complete before the barriers below:
+        // a significant sample's mask byte is written again, possibly by
+        // another wave, after them.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        seen = block_max(seen);   // (its barriers also publish the histogram)
+        if (!bound_max || attempt > 0 || seen == lmax) break;
+        lmax = seen;
+        any = lmax > -DBL_MAX;
     }
-    __syncthreads();
+    const bool listed = s_ncap <= (unsigned)kWCap;   // every positive weight is in the list
 
     // ---- first level: total, bound, kept bucket
     SelState st{u192_zero(), 0ull, false, 0u};
@@ -397,7 +444,36 @@ __global__ __launch_bounds__(kWT, 2) void weights_kernel(const WeightsArgs a)
     }
     const int eff_e = exponent_of(st.lo);
 
-    if (!all_pass && cnt0 <= (unsigned)kWCap) {
+    if (!all_pass && listed) {
+        // ---- the last two levels and the positive samples' mask bytes from
+        // the list (every other sample has weight 0 < w*, mask 0 from pass 2)
+        const int ncap = (int)s_ncap;
+        for (int level = 1; level < 3; ++level) {
+            const int shift = wshift(level);
+            const unsigned top = st.lo + ((1u << wshift(level - 1)) - 1u);
+            clear_hist();
+            __syncthreads();
+            for (int j = tid; j < ncap; j += kWT) {
+                const unsigned bits = __float_as_uint(cap_val[j]);
+                if (bits >= st.lo && bits <= top) {
+                    atomicAdd(&h_sum[(bits - st.lo) >> shift], significand(bits));
+                    atomicAdd(&h_cnt[(bits - st.lo) >> shift], 1u);
+                }
+            }
+            __syncthreads();
+            pick = select_level(level, h_sum[tid], h_cnt[tid], eff_e, a.fraction, bound, total,
+                                npos, st, ss, fallback);
+            st.below = ss.excl;
+            st.nbelow = ss.excl_cnt;
+            st.crossed = ss.cond;
+            st.lo += (unsigned)pick << shift;
+        }
+        if (mk) {
+            const float ws = __uint_as_float(st.lo);
+            for (int j = tid; j < ncap; j += kWT)
+                if (cap_val[j] >= ws) mk[cap_idx[j]] = 1;
+        }
+    } else if (!all_pass && cnt0 <= (unsigned)kWCap) {
         // ---- pass 3: mask bytes of the samples outside the kept bucket;
         // the samples inside it go to LDS
         const unsigned lo = st.lo, hi = st.lo + (1u << wshift(0));

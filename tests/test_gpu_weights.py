@@ -181,3 +181,31 @@ def test_weights_crowded_buckets_and_alignment(lib, torch_cuda):
         for k in ("weights", "sum", "significant_weight", "n_significant"):
             assert torch.equal(ref[k], nomask[k]), (fraction, k)
         assert wbuf[0].item() == -1.0 and mbuf[:3].tolist() == [7, 7, 7]
+
+
+def test_weights_minimum_not_attained(lib, torch_cuda):
+    """Without priors the kernel takes min - stored(min) as the largest log
+    weight and repeats its weights pass when no sample attains it: a minimum
+    below every sample (largest log weight smaller than the bound) and one
+    above some samples (those get weight 0), each against the oracle, next to
+    the minimum the search reports."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob = synth.make_problem("c1", seed=13, particles=4)
+    t = _search(lib, torch, prob)
+    best = t["best_diff2"].clone()
+    d = t["diff2"].cpu().numpy()
+    off = np.asarray(prob.trans_offset, np.int64)
+    O = prob.orientation_num
+    for shift in (0.0, -0.37, 2.5):
+        m = best + shift
+        out = lib.coarse_weights(prob.trans_offset, t["trans_offset"], 1, O, t["diff2"], m,
+                                 fraction=0.999)
+        torch.cuda.synchronize()
+        _check_arrays(d, m.cpu().numpy(), off, 1, O, out, 0.999, None, None)
+        if shift > 0:   # samples under the minimum are excluded
+            w = out["weights"].cpu().numpy()
+            for i in range(prob.n_particles):
+                a, b = O * off[i], O * off[i + 1]
+                below = d[a:b] < np.float32(m[i].item())
+                assert below.any() and np.all(w[a:b][below] == 0.0)
