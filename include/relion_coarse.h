@@ -110,7 +110,15 @@ int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ
  * call needing more than that stream has used before, allocates (and frees
  * the smaller buffer after synchronising that stream): such calls must not
  * be captured into a hipGraph.  All other calls neither allocate nor
- * synchronise.  rc_scratch_release frees a stream's buffer. */
+ * synchronise.  rc_scratch_release frees a stream's buffer.
+ *
+ * Projection cache: the scratch of a stream also keeps the projections of
+ * the last call (orientation_num x image_size complex floats, at most 1 GiB
+ * per stream; larger searches gather every call).  A call with the same
+ * projector, eulers pointer, sizes and image geometry reuses them after a
+ * device-side check that the eulers' contents are bit-identical to those
+ * the projections were made from (no host synchronisation); anything else
+ * makes them again.  The results are bit-identical either way. */
 int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
                                const float *d_trans_x, const float *d_trans_y,
                                const float *d_trans_z, const float *d_corr_img,

@@ -129,6 +129,13 @@ static SingleShape single_shape(int T, int O, int npix, int form)
     return sh;
 }
 
+// Device buffers of the drop-in's projection cache (see ScratchEntry).
+struct ProjCacheView {
+    float *eulers;   // copy of the eulers the projections were made from
+    int *flag;       // set by the check kernel when they must be made again
+    float2 *P;       // [O][kpad]
+};
+
 struct SingleWs {
     float2 *xw, *coords;
     double *xnorm;
@@ -161,7 +168,8 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                       const float *trans_x, const float *trans_y, const float *real,
                       const float *imag, const rc_projector_kernel *pk, const float *corr,
                       const float *init, float *dest, hipStream_t stream, const float *coor_xy,
-                      void *ws, size_t ws_bytes, bool cc = false)
+                      void *ws, size_t ws_bytes, bool cc = false,
+                      const ProjCacheView *cache = nullptr, int cache_force = 0)
 {
     if (int st = check_kernel_view(pk)) return st;
     RC_REQUIRE(T > 0 && O > 0 && npix > 0, RC_ERR_INVALID,
@@ -223,6 +231,13 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
     a.kc_per_split = sh.kc_per_split;
     a.part_c = w.part_c;
     a.part_pn = w.part_pn;
+    a.pcache = nullptr;
+    if (cache) {
+        RC_HIP(launch_cache_check(eulers, cache->eulers, 9 * O, cache_force, cache->flag, stream));
+        RC_HIP(launch_cache_fill(cache->flag, v, eulers, O, w.coords, npix, kpad, cache->P,
+                                 stream));
+        a.pcache = cache->P;
+    }
     RC_HIP(launch_cross(form, true, vi, a, stream));
     if (sh.ksplit > 1)
         RC_HIP(launch_splitk_assemble(O, T, sh.ksplit, w.part_c, w.part_pn, w.xnorm, init, dest,
@@ -259,16 +274,44 @@ int run_single_ws(int T, int O, int npix, int full_npix, int form, const float *
 // so a call that grows the buffer cannot free it under another thread that
 // is still launching with it; the old buffer is freed only after the stream
 // has drained.
+// The entry also keeps the drop-in's projection cache: RELION calls the
+// drop-in once per particle with one projector and one orientation set, so
+// the projections of a call are those of the last one unless the key
+// changed.  The host compares what it can see (projector, its generation,
+// sizes, image geometry, the eulers pointer); a device kernel compares the
+// eulers' contents with a copy, so a buffer refilled with other
+// orientations is caught without synchronising; the fill kernel recomputes
+// the projections only when either differs.
+struct ProjCacheKey {
+    const rc_projector *proj = nullptr;
+    unsigned long long generation = 0;
+    const float *eulers = nullptr;
+    int O = 0, npix = 0, imgX = 0, imgY = 0, maxR = 0, maxR2 = 0;
+    bool operator==(const ProjCacheKey &o) const
+    {
+        return proj == o.proj && generation == o.generation && eulers == o.eulers && O == o.O &&
+               npix == o.npix && imgX == o.imgX && imgY == o.imgY && maxR == o.maxR &&
+               maxR2 == o.maxR2;
+    }
+};
 struct ScratchEntry {
     std::mutex mu;
     void *ptr = nullptr;
     size_t bytes = 0;
+    // projection cache: [eulers copy][flag][P: O x kpad float2]
+    void *cache = nullptr;
+    size_t cache_bytes = 0;
+    ProjCacheKey key;
+    bool cache_valid = false;
 };
+// Largest projection cache per stream (float2 per orientation and pixel);
+// larger searches gather their projections every call.
+constexpr size_t kProjCacheMaxBytes = 1ull << 30;
 static std::mutex g_scratch_mu;
 static std::map<std::pair<int, hipStream_t>, std::unique_ptr<ScratchEntry>> g_scratch;
 
 static int scratch_for(hipStream_t stream, size_t bytes, std::unique_lock<std::mutex> &held,
-                       void **out)
+                       void **out, ScratchEntry **entry = nullptr)
 {
     int dev = 0;
     RC_HIP(hipGetDevice(&dev));
@@ -297,6 +340,42 @@ static int scratch_for(hipStream_t stream, size_t bytes, std::unique_lock<std::m
         e->bytes = want;
     }
     *out = e->ptr;
+    if (entry) *entry = e;
+    return RC_OK;
+}
+
+// Projection-cache buffers of a held scratch entry for O orientations of
+// kpad pixels (grown like the scratch: the old buffer is freed after the
+// stream drains).  *force is set when the cache holds nothing for this key.
+static int proj_cache_for(ScratchEntry *e, hipStream_t stream, const ProjCacheKey &key, int kpad,
+                          ProjCacheView *view, int *force)
+{
+    const size_t eb = align_up((size_t)9 * key.O * sizeof(float), 256);
+    const size_t need = eb + 256 + (size_t)key.O * kpad * sizeof(float2);
+    if (e->cache_bytes < need) {
+        if (e->cache) {
+            RC_HIP(hipStreamSynchronize(stream));
+            RC_HIP(hipFree(e->cache));
+            e->cache = nullptr;
+            e->cache_bytes = 0;
+        }
+        const size_t want = align_up(need, 1 << 20);
+        void *p = nullptr;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            set_error("hipMalloc of %zu projection-cache bytes failed", want);
+            return RC_ERR_NO_MEMORY;
+        }
+        e->cache = p;
+        e->cache_bytes = want;
+        e->cache_valid = false;
+    }
+    unsigned char *b = static_cast<unsigned char *>(e->cache);
+    view->eulers = reinterpret_cast<float *>(b);
+    view->flag = reinterpret_cast<int *>(b + eb);
+    view->P = reinterpret_cast<float2 *>(b + eb + 256);
+    *force = !(e->cache_valid && e->key == key);
+    e->key = key;
+    e->cache_valid = true;
     return RC_OK;
 }
 
@@ -313,10 +392,9 @@ static int scratch_release(hipStream_t stream)
         g_scratch.erase(it);
     }
     std::lock_guard<std::mutex> lk(e->mu);
-    if (e->ptr) {
-        RC_HIP(hipStreamSynchronize(stream));
-        RC_HIP(hipFree(e->ptr));
-    }
+    if (e->ptr || e->cache) RC_HIP(hipStreamSynchronize(stream));
+    if (e->ptr) RC_HIP(hipFree(e->ptr));
+    if (e->cache) RC_HIP(hipFree(e->cache));
     return RC_OK;
 }
 
@@ -698,7 +776,9 @@ int rc_projector_create(int xdim, int ydim, int zdim, int inity, int initz, int 
         tmp[i] = make_float2(host_real[i], host_imag[i]);
         m = std::max(m, std::max(std::fabs(host_real[i]), std::fabs(host_imag[i])));
     }
+    static std::atomic<unsigned long long> s_generation{0};
     rc_projector *p = new rc_projector{};
+    p->generation = ++s_generation;
     p->xdim = xdim;
     p->ydim = ydim;
     p->zdim = zdim;
@@ -865,11 +945,32 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
                                                (int)image_size, RC_FORM_FP32);
     std::unique_lock<std::mutex> held;
     void *ws = nullptr;
-    if (int st = scratch_for((hipStream_t)stream, need, held, &ws)) return st;
+    ScratchEntry *entry = nullptr;
+    if (int st = scratch_for((hipStream_t)stream, need, held, &ws, &entry)) return st;
+    const int kpad = (int)align_up((size_t)image_size, kChunk);
+    ProjCacheView cache{};
+    int force = 0;
+    const bool use_cache = (size_t)orientation_num * kpad * sizeof(float2) <= kProjCacheMaxBytes &&
+                           orientation_num <= 65535 && d_eulers;
+    if (use_cache) {
+        ProjCacheKey key;
+        key.proj = projector->proj;
+        key.generation = projector->proj->generation;
+        key.eulers = d_eulers;
+        key.O = (int)orientation_num;
+        key.npix = (int)image_size;
+        key.imgX = projector->imgX;
+        key.imgY = projector->imgY;
+        key.maxR = projector->maxR;
+        key.maxR2 = projector->maxR2_padded;
+        if (int st = proj_cache_for(entry, (hipStream_t)stream, key, kpad, &cache, &force))
+            return st;
+    }
     return run_single((int)translation_num, (int)orientation_num, (int)image_size,
                       (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real,
                       d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
-                      (hipStream_t)stream, nullptr, ws, need, do_CC != 0);
+                      (hipStream_t)stream, nullptr, ws, need, do_CC != 0,
+                      use_cache ? &cache : nullptr, force);
 }
 
 int rc_scratch_release(void *stream) { return scratch_release((hipStream_t)stream); }

@@ -313,6 +313,57 @@ __global__ __launch_bounds__(256) void splitk_assemble_kernel(int O, int T, int 
     out[k] = (float)d;
 }
 
+// One workgroup: compares the n floats bit for bit (a reused buffer with
+// new contents is a new key), then refreshes the copy if anything differs.
+__global__ __launch_bounds__(256) void cache_check_kernel(const float *__restrict__ eulers,
+                                                          float *__restrict__ copy, int n,
+                                                          int force, int *flag)
+{
+    __shared__ int s_diff;
+    if (threadIdx.x == 0) s_diff = force;
+    __syncthreads();
+    int diff = 0;
+    for (int i = threadIdx.x; i < n; i += 256)
+        diff |= __float_as_uint(eulers[i]) != __float_as_uint(copy[i]);
+    if (diff) s_diff = 1;
+    __syncthreads();
+    const int d = s_diff;
+    if (d)
+        for (int i = threadIdx.x; i < n; i += 256) copy[i] = eulers[i];
+    if (threadIdx.x == 0) *flag = d;
+}
+
+hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, int *flag,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL(cache_check_kernel, dim3(1), dim3(256), 0, s, eulers, copy, n, force, flag);
+    return hipGetLastError();
+}
+
+// block = 256 pixels of one orientation; the same sample as the fused
+// kernels' gather (rows e0, e1, e3, e4, e6, e7, pixel coordinates from prep)
+__global__ __launch_bounds__(256) void cache_fill_kernel(const int *flag, const ProjView v,
+                                                         const float *__restrict__ eulers,
+                                                         const float2 *__restrict__ coords,
+                                                         int npix, int kpad, float2 *P)
+{
+    if (*flag == 0) return;
+    const int o = blockIdx.y;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npix) return;
+    const float *e = eulers + 9 * (size_t)o;
+    const float2 xy = coords[p];
+    P[(size_t)o * kpad + p] = project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+}
+
+hipError_t launch_cache_fill(const int *flag, const ProjView &v, const float *eulers, int O,
+                             const float2 *coords, int npix, int kpad, float2 *P, hipStream_t s)
+{
+    hipLaunchKernelGGL(cache_fill_kernel, dim3((npix + 255) / 256, O), dim3(256), 0, s, flag, v,
+                       eulers, coords, npix, kpad, P);
+    return hipGetLastError();
+}
+
 hipError_t launch_splitk_assemble(int O, int T, int S, const float *part_c, const float *part_pn,
                                   const double *xnorm, const float *init, float *out, int cc,
                                   hipStream_t s)

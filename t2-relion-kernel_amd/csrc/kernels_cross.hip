@@ -183,9 +183,13 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
                 const int p = kc * kChunk + pl;
                 float2 val = make_float2(0.f, 0.f);
                 if (o0 + col < O && p < a.npix) {
-                    const float2 xy = a.coords[p];
-                    const float *e = sE + col * 6;
-                    val = project_sample(vol, e[0], e[1], e[2], e[3], e[4], e[5], xy.x, xy.y);
+                    if (a.pcache) {
+                        val = a.pcache[(size_t)(o0 + col) * a.kpad + p];
+                    } else {
+                        const float2 xy = a.coords[p];
+                        const float *e = sE + col * 6;
+                        val = project_sample(vol, e[0], e[1], e[2], e[3], e[4], e[5], xy.x, xy.y);
+                    }
                 }
                 pn_acc[s] = fmaf(a.half_corr[p], val.x * val.x + val.y * val.y, pn_acc[s]);
                 store_encoded<FORM>(sB + col * ROW + (pl >> 3) * GB, pl & 7, val.x, val.y, bsc);

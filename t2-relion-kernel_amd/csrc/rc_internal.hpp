@@ -99,6 +99,10 @@ struct CrossArgs {
     int ksplit, kc_per_split;
     float *part_c;               // [ksplit][O][n_trans] cross-term partials
     float *part_pn;              // [ksplit][O] sum corr/2 |P|^2 partials
+    // fused kernels: projections computed by an earlier call (drop-in
+    // projection cache), [O][kpad] float2, read instead of the trilinear
+    // gather; NULL = gather
+    const float2 *pcache;
     // wide kernels with the translation operand precomputed (schedule 8):
     // [row tile][chunk][tile_m rows][row bytes], each chunk block the exact
     // (swizzled) LDS image of the tile's A buffer; a_chunks = chunks per row tile
@@ -183,6 +187,14 @@ hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half
 
 // Split-K assembly of one image: out[o*T+t] = score from the fixed-order
 // float64 sums over the S slices of part_c and part_pn (see CrossArgs).
+// Drop-in projection cache.  cache_check: flag = force, or the eulers
+// differ bit for bit from the cached copy; when set, the copy is refreshed.
+// cache_fill: when the flag is set, P[o][p] = the projection of orientation o
+// at pixel p (p < npix; the fused gather's arithmetic), else nothing.
+hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, int *flag,
+                              hipStream_t s);
+hipError_t launch_cache_fill(const int *flag, const ProjView &v, const float *eulers, int O,
+                             const float2 *coords, int npix, int kpad, float2 *P, hipStream_t s);
 hipError_t launch_splitk_assemble(int O, int T, int S, const float *part_c, const float *part_pn,
                                   const double *xnorm, const float *init, float *out, int cc,
                                   hipStream_t s);
@@ -261,4 +273,5 @@ struct rc_projector {
     float *d_scale;     // device float: power-of-two scale of max_abs (FP16 form)
     int device;
     size_t bytes;
+    unsigned long long generation;   // unique per created projector (cache keys)
 };

@@ -142,6 +142,52 @@ def test_drop_in_on_concurrent_streams(c2p, lib, torch_cuda):
         lib.scratch_release(s)
 
 
+def test_drop_in_projection_cache(c2p, lib, torch_cuda):
+    """The drop-in keeps, per stream, the projections of its last call: calls
+    with the same projector and orientations reuse them and give the bits of
+    the allocation-free entry point, which gathers on every call; new
+    orientations written into the same buffer, and another projector, are
+    caught and give that entry point's bits for them."""
+    from relion_amd import synth
+    torch = torch_cuda
+    prob, _ = c2p
+    pb, g = prob.particles, prob.geom
+    inv = _inverse(synth.filter_order(g)[0])
+    O, i = prob.orientation_num, 3
+    tx, ty = prob.trans_of(i)
+    ins = [_t(torch, a) for a in (tx, ty, pb.corr[i][inv], pb.real[i][inv], pb.imag[i][inv])]
+    import dataclasses
+    v0 = prob.vols[0]
+    half = dataclasses.replace(v0, real=v0.real * np.float32(0.5), imag=v0.imag * np.float32(0.5))
+    projs = [lib.Projector(v0), lib.Projector(half)]
+    pks = [p.kernel(g.imgX, g.imgY, g.maxR) for p in projs]
+    eul = _t(torch, prob.eulers.ravel())
+    s = torch.cuda.Stream()
+
+    def run(pk, fn, stream=None):
+        d = torch.zeros(O * 113, dtype=torch.float32, device="cuda")
+        fn(pk, *ins, eul, d, O, 113, g.image_size, stream=stream)
+        torch.cuda.synchronize()
+        return d
+
+    cached = lambda pk: run(pk, lib.run_diff2_kernel_coarse, s)
+    gathered = lambda pk: run(pk, lib.run_diff2_kernel_coarse_noalloc)
+    ref = gathered(pks[0])
+    for k in range(3):
+        assert torch.equal(cached(pks[0]), ref), k
+    # other orientations in the same buffer: the eulers rotated by one row
+    eul.copy_(_t(torch, np.roll(prob.eulers, 1, axis=0).ravel()))
+    ref2 = gathered(pks[0])
+    assert not torch.equal(ref2, ref)
+    assert torch.equal(cached(pks[0]), ref2)
+    assert torch.equal(cached(pks[0]), ref2)
+    # another projector, then the first one again
+    ref3 = gathered(pks[1])
+    assert torch.equal(cached(pks[1]), ref3)
+    assert torch.equal(cached(pks[0]), ref2)
+    lib.scratch_release(s)
+
+
 def test_filter_images_on_device(lib, torch_cuda):
     """rc_filter_coarse_images reproduces the host order exactly: the test
     problems' filtered images from their unfiltered form, and random data
