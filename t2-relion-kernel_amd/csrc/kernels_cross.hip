@@ -172,27 +172,57 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
         }
     }
 
+    // Fused kernels with cached projections (a.pcache): a chunk's values --
+    // thread tid holds pixel tid % 32 of columns tid / 32 + 8 s -- and its
+    // pixel weight are loaded a chunk ahead, under the previous chunk's MFMAs.
+    static_assert(!FUSED || NT % kChunk == 0, "one pixel per thread in every chunk");
+    float2 pv[FUSED ? SPT : 1];
+    float hcv = 0.f;
+    auto load_cached = [&](int kc) {
+        const int pl = tid & (kChunk - 1);
+        const int p = kc * kChunk + pl;
+        hcv = a.half_corr[p];
+#pragma unroll
+        for (int s = 0; s < SPT; ++s) {
+            const int col = (tid + s * NT) >> 5;
+            pv[s] = (o0 + col < O && p < a.npix) ? a.pcache[(size_t)(o0 + col) * a.kpad + p]
+                                                 : make_float2(0.f, 0.f);
+        }
+    };
+    if constexpr (FUSED) {
+        if (a.pcache && kc0 < kc1) load_cached(kc0);
+    }
+
     for (int kc = kc0; kc < kc1; ++kc) {
         __syncthreads();
         if constexpr (FUSED) {
-            const ProjView vol = a.vol[0];
+            if (a.pcache) {
+#pragma unroll
+                for (int s = 0; s < SPT; ++s) {
+                    const int idx = tid + s * NT;
+                    const int col = idx >> 5, pl = idx & 31;
+                    const float2 val = pv[s];
+                    pn_acc[s] = fmaf(hcv, val.x * val.x + val.y * val.y, pn_acc[s]);
+                    store_encoded<FORM>(sB + col * ROW + (pl >> 3) * GB, pl & 7, val.x, val.y,
+                                        bsc);
+                }
+            } else {
+                const ProjView vol = a.vol[0];
 #pragma unroll 2
-            for (int s = 0; s < SPT; ++s) {
-                const int idx = tid + s * NT;
-                const int col = idx >> 5, pl = idx & 31;
-                const int p = kc * kChunk + pl;
-                float2 val = make_float2(0.f, 0.f);
-                if (o0 + col < O && p < a.npix) {
-                    if (a.pcache) {
-                        val = a.pcache[(size_t)(o0 + col) * a.kpad + p];
-                    } else {
+                for (int s = 0; s < SPT; ++s) {
+                    const int idx = tid + s * NT;
+                    const int col = idx >> 5, pl = idx & 31;
+                    const int p = kc * kChunk + pl;
+                    float2 val = make_float2(0.f, 0.f);
+                    if (o0 + col < O && p < a.npix) {
                         const float2 xy = a.coords[p];
                         const float *e = sE + col * 6;
                         val = project_sample(vol, e[0], e[1], e[2], e[3], e[4], e[5], xy.x, xy.y);
                     }
+                    pn_acc[s] = fmaf(a.half_corr[p], val.x * val.x + val.y * val.y, pn_acc[s]);
+                    store_encoded<FORM>(sB + col * ROW + (pl >> 3) * GB, pl & 7, val.x, val.y,
+                                        bsc);
                 }
-                pn_acc[s] = fmaf(a.half_corr[p], val.x * val.x + val.y * val.y, pn_acc[s]);
-                store_encoded<FORM>(sB + col * ROW + (pl >> 3) * GB, pl & 7, val.x, val.y, bsc);
             }
         } else {
 #pragma unroll
@@ -205,6 +235,9 @@ __global__ __launch_bounds__(64 * WM * WN) void cross_kernel(const CrossArgs a)
             }
         }
         __syncthreads();
+        if constexpr (FUSED) {
+            if (a.pcache && kc + 1 < kc1) load_cached(kc + 1);
+        }
         if constexpr (!FUSED) {
             if (kc + 1 < nkc) {
                 const unsigned char *src = a.proj + (size_t)cls * O * nkc * CB;
