@@ -535,7 +535,7 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     // padding: the wide kernels' copies of a last, partial column tile run
     // up to one tile (512 columns x 128 bytes) past the last projection
     p->proj = c.take<unsigned char>(
-        (p->info.wide ? projection_bytes(p->form, p->info.schedule == kScheduleDirect, p->C,
+        (p->info.wide ? projection_bytes(p->form, (p->info.schedule & kScheduleDirect) != 0, p->C,
                                          p->O, p->kpad)
                       : (size_t)p->C * p->O * p->nkc * chunk_bytes(p->form)) +
         kProjPad);
@@ -544,11 +544,12 @@ size_t plan_bind(rc_plan *p, unsigned char *base)
     const size_t prows = p->packed ? (size_t)p->n_rows : (size_t)p->n_groups;
     p->part_min = c.take<double>(prows * nt_total * p->parts_per_group);
     p->part_idx = c.take<long long>(prows * nt_total * p->parts_per_group);
-    // schedule 8: the translation operand of every row tile and chunk, in
-    // the LDS image of the kernel's A buffer (tile_m rows x row bytes)
+    // schedule 8 (wide or direct): the translation operand of every row
+    // tile and chunk, in the LDS image of the kernel's A buffer (tile_m rows
+    // x row bytes)
     p->a_chunks = (p->npix + 15) / 16;
     p->a_enc = nullptr;
-    if (p->info.wide && p->info.schedule == 8)
+    if (p->info.wide && (p->info.schedule & kSchedulePrecomputedA))
         p->a_enc = c.take<unsigned char>((size_t)p->n_mtiles * p->a_chunks * p->info.tile_m *
                                          (p->form == kFP16 ? 64 : 128));
     return align_up(c.off, 256);
@@ -692,7 +693,7 @@ int plan_run(rc_plan *p, const rc_batch_desc *d, hipStream_t s)
     if (p->info.wide)
         RC_HIP(launch_project_wide(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O,
                                    p->coords, p->npix, p->kpad, p->proj, p->pabs2,
-                                   p->info.schedule == kScheduleDirect, s));
+                                   (p->info.schedule & kScheduleDirect) != 0, s));
     else
         RC_HIP(launch_project(p->form, p->vols, p->bscale, p->C, d->d_eulers, p->O, p->coords,
                               p->npix, p->kpad, p->proj, p->pabs2, s));
@@ -1198,6 +1199,8 @@ const char *rc_variant_name(int form, int variant)
              : v->schedule == 4 ? ", fenced segments"
              : v->schedule == 8 ? ", translation operand precomputed"
              : v->schedule == kScheduleDirect ? ", projections loaded into registers"
+             : v->schedule == kScheduleDirectPre
+                 ? ", projections loaded into registers, translation operand precomputed"
                                 : "");
     return buf;
 }

@@ -28,6 +28,13 @@
 // kernels do) but only 16 KB of LDS writes and 64 KB of LDS reads (those
 // kernels: 80 KB and 192 KB).
 //
+// PRE instances take the translation operand precomputed (encode_a_kernel:
+// each row tile's chunk is one contiguous 16 KB block holding the exact LDS
+// image of the A buffer) and copy it global -> LDS with LDS-DMA, two chunks
+// ahead: no image staging, no phase arithmetic in the loop.  The copy of
+// A(k+2) goes out right after step k's barrier (which frees A(k)'s buffer)
+// and is waited for at step k+1's barrier, one and a half steps later.
+//
 // The MFMA sequence of every accumulator -- chunk by chunk, and within a
 // chunk (lo x hi, hi x lo, hi x hi) for BF16X3 -- and the operand values are
 // those of the wide kernels, so the results are bit-identical to theirs.
@@ -93,7 +100,15 @@ struct EpiTables {
 
 }  // namespace
 
-template <int FORM>
+// s_waitcnt vmcnt(N), then the block barrier (LDS reads drained first).
+template <int N>
+__device__ __forceinline__ void wait_vm_barrier()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    asm volatile("s_waitcnt vmcnt(%0)\n\ts_waitcnt lgkmcnt(0)\n\ts_barrier" ::"n"(N) : "memory");
+}
+
+template <int FORM, bool PRE>
 __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(1, 1)))
 void cross_kernel_direct(const CrossArgs a)
 {
@@ -107,7 +122,7 @@ void cross_kernel_direct(const CrossArgs a)
     static_assert(TPR == 2 && PPT == FN, "one pixel of the build per column slice");
 
     __shared__ __attribute__((aligned(16))) unsigned char sA[2][TM * KB];
-    __shared__ __attribute__((aligned(16))) unsigned char sImg[kStageSlots][kStageBytes];
+    __shared__ __attribute__((aligned(16))) unsigned char sImg[kStageSlots][PRE ? 16 : kStageBytes];
     // epilogue inputs (EpiTables)
     __shared__ __attribute__((aligned(16))) float s_pn[kDirectMaxTileParts * TN];
     __shared__ double s_xn[kDirectMaxTileParts];
@@ -191,7 +206,7 @@ void cross_kernel_direct(const CrossArgs a)
 
     WideFrag<FORM> aR[FM], bR[FN];
     float4 xw[PPT / 2], cxy[PPT / 2];
-    float2 v[PPT];
+    APix<FORM> v[PPT];
 
     // pixels 2u, 2u + 1 of the thread's eight in staged chunk c
     auto read_img = [&](int c, int u) {
@@ -204,34 +219,68 @@ void cross_kernel_direct(const CrossArgs a)
         wide_load<FORM>(aR[fa], buf + ra * KB, ra, q);
     };
 
+    // ---- PRE: A chunk kc of this row tile, copied into A buffer buf.
+    // Thread t copies bytes 16 t + 4096 i of the 16 KB block (i < LPA): each
+    // wave writes 1 KB runs of LDS from its own base, as the copy requires.
+    constexpr int LPA = TM * KB / (NT * 16);
+    static_assert(LPA * NT * 16 == TM * KB, "A chunk splits evenly into 16-byte copies");
+    constexpr int LB = FORM == kFP16 ? 1 : 2;   // global loads per column slice of B
+    const unsigned sA_lds = uniform32(
+        (unsigned)(size_t)(__attribute__((address_space(3))) void *)&sA[0][0] + wn * 64 * 16);
+    const unsigned long long a_tile = uniform64(
+        (unsigned long long)(size_t)(a.a_enc + (size_t)mtile * a.a_chunks * (TM * KB)));
+    const uint32_t a_off = (uint32_t)(tid * 16);
+    auto issue_A = [&](int kc, int buf) {
+        const unsigned long long chunk =
+            a_tile + (unsigned long long)(unsigned)kc * (unsigned)(TM * KB);
+#pragma unroll
+        for (int it = 0; it < LPA; ++it)
+            lds_copy16(chunk + (unsigned long long)it * NT * 16, a_off,
+                       sA_lds + buf * (TM * KB) + it * NT * 16);
+    };
+
     // ---- prologue: image data of chunks 0-3 staged; A(0) and A(1) built
-    // and stored; B(0) loaded; A(0) read after the barrier.  The loads and
-    // the LDS reads go out in the order the loop issues them, so that the
-    // waits the compiler places at the top of the loop (valid on entry and
-    // on every later step) are those of the steady state.
-    {
-        float4 st[kStageSlots];
-#pragma unroll
-        for (int c = 0; c < kStageSlots; ++c) st[c] = stage_load(c);
-#pragma unroll
-        for (int c = 0; c < kStageSlots; ++c) stage_store(c, st[c]);
-    }
-    __syncthreads();
-    for (int c = 0; c < 2; ++c) {
-#pragma unroll
-        for (int u = 0; u < PPT / 2; ++u) read_img(c, u);
-        a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[c][gr * KB]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    static_for<0, FN>([&](auto fb_c) {
-        constexpr int fb = decltype(fb_c)::value;
-        direct_load_b<FORM>(bR[fb], b_wave + fb * GB, lane);
+    // and stored (PRE: copied); B(0) loaded; A(0) read after the barrier.
+    // The loads and the LDS reads go out in the order the loop issues them,
+    // so that the waits the compiler places at the top of the loop (valid
+    // on entry and on every later step) are those of the steady state.
+    auto load_b0 = [&]() {
+        static_for<0, FN>([&](auto fb_c) {
+            constexpr int fb = decltype(fb_c)::value;
+            direct_load_b<FORM>(bR[fb], b_wave + fb * GB, lane);
+            __builtin_amdgcn_sched_barrier(0);
+        });
+    };
+    float4 stg{};
+    if constexpr (PRE) {
+        issue_A(0, 0);
+        issue_A(min(1, nrun - 1), 1);
         __builtin_amdgcn_sched_barrier(0);
-    });
-    float4 stg = stage_load(kStageSlots);
-    read_img(2, 0);
-    read_img(2, 1);
-    __syncthreads();
+        load_b0();
+        // A(0) landed: everything but A(1) and B(0); the barrier also
+        // publishes the epilogue tables
+        wait_vm_barrier<LPA + FN * LB>();
+    } else {
+        {
+            float4 st[kStageSlots];
+#pragma unroll
+            for (int c = 0; c < kStageSlots; ++c) st[c] = stage_load(c);
+#pragma unroll
+            for (int c = 0; c < kStageSlots; ++c) stage_store(c, st[c]);
+        }
+        __syncthreads();
+        for (int c = 0; c < 2; ++c) {
+#pragma unroll
+            for (int u = 0; u < PPT / 2; ++u) read_img(c, u);
+            a_build<FORM, PPT>(ar, xw, cxy, gh, gr, &sA[c][gr * KB]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        load_b0();
+        stg = stage_load(kStageSlots);
+        read_img(2, 0);
+        read_img(2, 1);
+        __syncthreads();
+    }
     if constexpr (FORM == kBF16X3) {
 #pragma unroll
         for (int fa = 0; fa < FM; ++fa) {
@@ -266,7 +315,7 @@ void cross_kernel_direct(const CrossArgs a)
             // ahead of its use: pixels fb + 2, fb + 3 of chunk k+2 (staged two
             // steps ago, published by the last step's barrier); in slice 6
             // pixels 0, 1 of chunk k+3 (published by this step's barrier)
-            if constexpr ((fb & 1) == 0) {
+            if constexpr (!PRE && (fb & 1) == 0) {
                 if constexpr (fb == FN - 2) read_img(k + 3, 0);
                 else read_img(k + 2, (fb + 2) >> 1);
             }
@@ -317,23 +366,42 @@ void cross_kernel_direct(const CrossArgs a)
                     for (int fa = 0; fa < FM; ++fa) read_a(s_next, fa);
                 }
             }
-            // pixel fb of chunk k+2
-            v[fb] = a_pixel<PPT>(ar, xw, cxy, fb);
+            // pixel fb of chunk k+2, encoded
+            if constexpr (!PRE) v[fb] = a_encode<FORM>(ar, a_pixel<PPT>(ar, xw, cxy, fb));
+            // PRE: A(k+2) into A(k)'s buffer, freed by this step's barrier;
+            // issued before slice 4's B load, so that the B loads the
+            // compiler waits for are never younger than a copy it cannot see
+            if constexpr (PRE && fb == 4) issue_A(min(k + 2, nrun - 1), k & 1);
             // chunk k+1 of this column slice, into the registers just used
             direct_load_b<FORM>(bR[fb], b_next + fb * GB, lane);
             if constexpr (fb == 3) {
                 // publishes A(k+1) and the image data of chunk k+4; every wave
                 // has finished reading A(k), whose buffer receives A(k+2) at
-                // the end of this step
-                asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+                // the end of this step.  PRE: A(k+1) was copied at slice 4 of
+                // the previous step; eight column slices of B loads followed it.
+                if constexpr (PRE) wait_vm_barrier<FN * LB>();
+                else asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+            }
+            if constexpr (FORM == kBF16X3 && !PRE) {
+                // One vector-ALU instruction after each MFMA: the build's
+                // float64 phase, sin/cos and encode run under the slice's 24
+                // MFMAs instead of bunching between its first few (and the
+                // accumulators stay put: no AGPR copies in the loop).  FP16
+                // (8 MFMAs per slice) is faster left to the scheduler.
+                static_for<0, 3 * FM>([&](auto) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                    __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);   // VALU
+                });
             }
             __builtin_amdgcn_sched_barrier(0);
         });
-        a_store<FORM, PPT>(ar, v, gh, gr, &s_build[gr * KB]);
-        // chunk k+4 into the slot of chunk k (read two steps ago), the next
-        // stage load in flight for a step
-        stage_store(k + kStageSlots, stg);
-        stg = stage_load(k + 1 + kStageSlots);
+        if constexpr (!PRE) {
+            a_store_enc<FORM, PPT>(v, gh, gr, &s_build[gr * KB]);
+            // chunk k+4 into the slot of chunk k (read two steps ago), the
+            // next stage load in flight for a step
+            stage_store(k + kStageSlots, stg);
+            stg = stage_load(k + 1 + kStageSlots);
+        }
     }
 
     static_assert(TN == 512, "EpiTables assumes 512 columns");
@@ -342,41 +410,44 @@ void cross_kernel_direct(const CrossArgs a)
                                         wn * FN * 16, r16, q, tables);
 }
 
-template <int FORM>
+template <int FORM, bool PRE>
 static hipError_t launch_direct_one(const CrossArgs &a, hipStream_t s)
 {
     const int total = a.n_mtiles * a.n_classes * a.ntiles_per_class;
     const int grid = ((total + 7) / 8) * 8;
-    hipLaunchKernelGGL(cross_kernel_direct<FORM>, dim3(grid), dim3(kDirThreads), 0, s, a);
+    if (PRE && (a.a_enc == nullptr || a.a_chunks != (a.npix + 15) / 16)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((cross_kernel_direct<FORM, PRE>), dim3(grid), dim3(kDirThreads), 0, s, a);
     return hipGetLastError();
 }
 
 // LDS bytes of one block: the double-buffered translation operand, the
-// image-data ring and the epilogue tables.
-template <int FORM>
+// image-data ring (not PRE) and the epilogue tables.
+template <int FORM, bool PRE>
 constexpr int direct_lds_bytes()
 {
-    return 2 * kDirTM * WideEnc<FORM>::kBytes + kStageSlots * kStageBytes +
+    return 2 * kDirTM * WideEnc<FORM>::kBytes + kStageSlots * (PRE ? 16 : kStageBytes) +
            kDirectMaxTileParts * (kDirWN * kDirFN * 16 * 4 + 8 + 3 * 4) + kDirTM * 4;
 }
 
-template <int FORM>
+template <int FORM, bool PRE>
 static VariantEntry direct_one()
 {
     return VariantEntry{VariantInfo{FORM, kDirTM, kDirWN * kDirFN * 16, kDirThreads, kDirWN,
-                                    false, direct_lds_bytes<FORM>(), false, true,
-                                    "cross_kernel_direct", kScheduleDirect,
+                                    false, direct_lds_bytes<FORM, PRE>(), false, true,
+                                    "cross_kernel_direct",
+                                    PRE ? kScheduleDirectPre : kScheduleDirect,
                                     kDirectMaxTileParts},
-                        &launch_direct_one<FORM>};
+                        &launch_direct_one<FORM, PRE>};
 }
 
-static const VariantEntry k_direct_fp32[] = {direct_one<kFP32>()};
-static const VariantEntry k_direct_bf16x3[] = {direct_one<kBF16X3>()};
-static const VariantEntry k_direct_fp16[] = {direct_one<kFP16>()};
+static const VariantEntry k_direct_fp32[] = {direct_one<kFP32, false>(), direct_one<kFP32, true>()};
+static const VariantEntry k_direct_bf16x3[] = {direct_one<kBF16X3, false>(),
+                                               direct_one<kBF16X3, true>()};
+static const VariantEntry k_direct_fp16[] = {direct_one<kFP16, false>(), direct_one<kFP16, true>()};
 
 const VariantEntry *direct_table(int form, int *n)
 {
-    *n = 1;
+    *n = 2;
     switch (form) {
     case kFP32: return k_direct_fp32;
     case kBF16X3: return k_direct_bf16x3;

@@ -128,13 +128,17 @@ struct VariantInfo {
     const char *family = "cross_kernel";   // kernel template name
     int schedule = 0;      // wide kernels: wave schedule (0 plain, 1/2 staggered, 4 fenced,
                            // 8 translation operand precomputed and copied like B,
-                           // kScheduleDirect projections loaded straight into registers)
+                           // kScheduleDirect projections loaded straight into registers,
+                           // kScheduleDirectPre both)
     int max_tile_parts = 0;  // > 0: at most this many particles in one row tile
 };
 
 // Schedule of the direct kernels (kernels_direct.hip), whose projections use
 // the direct layout (launch_project_wide, direct = true).
 constexpr int kScheduleDirect = 16;
+// Bit of the schedule: the translation operand is precomputed (a_enc).
+constexpr int kSchedulePrecomputedA = 8;
+constexpr int kScheduleDirectPre = kScheduleDirect | kSchedulePrecomputedA;
 // Particles one row tile of the direct kernels may hold (their image data
 // is staged in LDS per tile).
 constexpr int kDirectMaxTileParts = 31;

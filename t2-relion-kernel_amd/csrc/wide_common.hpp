@@ -132,33 +132,62 @@ __device__ __forceinline__ void a_store_words(const uint32_t (&hi)[PPT], const u
     }
 }
 
-// Encodes the thread's PPT (8 or 4) values -- pixels PPT*h .. PPT*h+PPT-1 of
-// the chunk -- into LDS row buffer `dst` of row r.
+// One translation-operand value in the words its form stores: BF16X3 the
+// (hi, lo) words of (re, im); FP16 the scaled (re, im) halves; FP32 the
+// floats themselves.  Encoding a value as soon as it is made (rather than
+// a chunk's values at once) lets the conversions run under the MFMAs.
+template <int FORM>
+struct APix {
+    float2 z;
+};
+template <>
+struct APix<kBF16X3> {
+    uint32_t hi, lo;
+};
+template <>
+struct APix<kFP16> {
+    uint32_t w;
+};
+
+template <int FORM>
+__device__ __forceinline__ APix<FORM> a_encode(const ARow &ar, float2 z)
+{
+    APix<FORM> e;
+    if constexpr (FORM == kBF16X3) {
+        encode_bf16x3(z, e.hi, e.lo);
+    } else if constexpr (FORM == kFP16) {
+        const _Float16 re = (_Float16)(z.x * ar.scale), im = (_Float16)(z.y * ar.scale);
+        e.w = (uint32_t)__builtin_bit_cast(uint16_t, re) |
+              ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
+    } else {
+        (void)ar;
+        e.z = z;
+    }
+    return e;
+}
+
+// Stores the thread's PPT (8 or 4) encoded values -- pixels PPT*h ..
+// PPT*h+PPT-1 of the chunk -- into LDS row buffer `dst` of row r.
 template <int FORM, int PPT>
-__device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], int h, int r,
-                                        unsigned char *dst)
+__device__ __forceinline__ void a_store_enc(const APix<FORM> (&e)[PPT], int h, int r,
+                                            unsigned char *dst)
 {
     static_assert(PPT == 8 || PPT == 4, "8 or 4 pixels per thread");
     constexpr int G = PPT / 4;   // 4-pixel groups (one 16-byte segment each)
     if constexpr (FORM == kBF16X3) {
         uint32_t hi[PPT], lo[PPT];
 #pragma unroll
-        for (int j = 0; j < PPT; ++j) encode_bf16x3(v[j], hi[j], lo[j]);
+        for (int j = 0; j < PPT; ++j) {
+            hi[j] = e[j].hi;
+            lo[j] = e[j].lo;
+        }
         a_store_words<PPT>(hi, lo, h, r, dst);
     } else if constexpr (FORM == kFP16) {
 #pragma unroll
-        for (int half = 0; half < G; ++half) {
-            uint32_t w[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const float2 z = v[4 * half + i];
-                const _Float16 re = (_Float16)(z.x * ar.scale), im = (_Float16)(z.y * ar.scale);
-                w[i] = (uint32_t)__builtin_bit_cast(uint16_t, re) |
-                       ((uint32_t)__builtin_bit_cast(uint16_t, im) << 16);
-            }
+        for (int half = 0; half < G; ++half)
             *reinterpret_cast<uint4 *>(dst + wide_swz<FORM>(r, G * h + half) * 16) =
-                make_uint4(w[0], w[1], w[2], w[3]);
-        }
+                make_uint4(e[4 * half].w, e[4 * half + 1].w, e[4 * half + 2].w,
+                           e[4 * half + 3].w);
     } else if constexpr (PPT == 8) {
         // segment 2q + h holds component q % 2 of pixels 2i + q / 2, i = 0..3
 #pragma unroll
@@ -166,7 +195,7 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], 
             float f[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const float2 z = v[2 * i + (q >> 1)];
+                const float2 z = e[2 * i + (q >> 1)].z;
                 f[i] = (q & 1) ? z.y : z.x;
             }
             *reinterpret_cast<float4 *>(dst + wide_swz<FORM>(r, 2 * q + h) * 16) =
@@ -177,12 +206,23 @@ __device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], 
         // and 2 + q / 2: 8 bytes in segment 2q + h/2 at byte (h & 1) * 8
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 z0 = v[q >> 1], z1 = v[2 + (q >> 1)];
+            const float2 z0 = e[q >> 1].z, z1 = e[2 + (q >> 1)].z;
             const float2 f = (q & 1) ? make_float2(z0.y, z1.y) : make_float2(z0.x, z1.x);
             *reinterpret_cast<float2 *>(dst + wide_swz<FORM>(r, 2 * q + (h >> 1)) * 16 +
                                         (h & 1) * 8) = f;
         }
     }
+}
+
+// Encodes the thread's PPT values and stores them (a_encode + a_store_enc).
+template <int FORM, int PPT>
+__device__ __forceinline__ void a_store(const ARow &ar, const float2 (&v)[PPT], int h, int r,
+                                        unsigned char *dst)
+{
+    APix<FORM> e[PPT];
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) e[j] = a_encode<FORM>(ar, v[j]);
+    a_store_enc<FORM, PPT>(e, h, r, dst);
 }
 
 // Builds the thread's part h of a chunk into LDS row buffer `dst` (row r).

@@ -44,7 +44,8 @@ def test_version_and_variants():
 def test_default_variant_indices_name_their_kernels():
     """The host picks defaults by table index: 14 (wide 256 x 256,
     staggered), 21 (wide 128 x 512, fenced), 23 (wide 256 x 256, operand
-    precomputed) and 24 (direct 128 x 512) must be those kernels."""
+    precomputed), 24 (direct 128 x 512) and 25 (direct 128 x 512, operand
+    precomputed) must be those kernels."""
     for form in ("fp32", "bf16x3", "fp16"):
         assert "256x256" in lib.variant_name(form, 14) and "staggered" in lib.variant_name(form, 14)
         assert "128x512" in lib.variant_name(form, 21) and "fenced" in lib.variant_name(form, 21)
@@ -52,6 +53,9 @@ def test_default_variant_indices_name_their_kernels():
         assert "precomputed" in lib.variant_name(form, 23)
         assert lib.variant_name(form, 24).startswith("cross_kernel_direct")
         assert "128x512" in lib.variant_name(form, 24)
+        assert "precomputed" not in lib.variant_name(form, 24)
+        assert lib.variant_name(form, 25).startswith("cross_kernel_direct")
+        assert "precomputed" in lib.variant_name(form, 25)
 
 
 def test_workspace_query():

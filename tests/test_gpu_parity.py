@@ -185,7 +185,7 @@ def _direct(lib, form):
             if lib.variant_name(form, v).startswith("cross_kernel_direct")]
 
 
-@pytest.mark.parametrize("kernel", ["default", "wide", "wide512", "direct"])
+@pytest.mark.parametrize("kernel", ["default", "wide", "wide512", "direct", "direct_pre"])
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
 def test_batched_cc(c1, lib, torch_cuda, form, kernel):
     """Batched cross-correlation scores; the per-particle minimum is the best
@@ -206,6 +206,8 @@ def test_batched_cc(c1, lib, torch_cuda, form, kernel):
             lib.set_variant_override(_wide_512(lib, form)[-1])
         elif kernel == "direct":
             lib.set_variant_override(_direct(lib, form)[0])
+        elif kernel == "direct_pre":
+            lib.set_variant_override(_direct(lib, form)[1])
         plan = lib.plan_for_problem(prob, projs, form, cc=True)
         t = lib.problem_tensors(prob)
         plan.execute(t)
@@ -443,7 +445,7 @@ def _plain_and_precomputed(lib, form):
     names = {v: lib.variant_name(form, v) for v, _, _ in lib.variants(form)}
     pairs = []
     for v, n in names.items():
-        if "translation operand precomputed" not in n:
+        if "translation operand precomputed" not in n or not n.startswith("cross_kernel_wide"):
             continue
         shape = n.split(",")[0]                 # "cross_kernel_wide<...> 128x512"
         plain = [u for u, m in names.items()
@@ -483,23 +485,26 @@ def test_precomputed_operand_is_bit_identical(c1, lib, torch_cuda, form):
 
 @pytest.mark.parametrize("form", ["fp32", "bf16x3", "fp16"])
 def test_direct_kernel_is_bit_identical(c1, lib, torch_cuda, form):
-    """The direct kernel (projections loaded straight into registers, one
-    wave per SIMD) builds the same operands and runs every accumulator's
-    MFMAs in the same order as the wide 128 x 512 kernels, so it reproduces
-    them bit for bit: the 32^2 workload (a partial last column tile of 64
-    orientations) and the ragged two-class batch (a partial last row tile)."""
+    """The direct kernels (projections loaded straight into registers, one
+    wave per SIMD; the translation operand built in the kernel, or
+    precomputed and copied) build the same operands and run every
+    accumulator's MFMAs in the same order as the wide 128 x 512 kernels, so
+    they reproduce them bit for bit: the 32^2 workload (a partial last column
+    tile of 64 orientations) and the ragged two-class batch (a partial last
+    row tile)."""
     torch = torch_cuda
     names = {v: lib.variant_name(form, v) for v, _, _ in lib.variants(form)}
     plain = [v for v, n in names.items()
              if n.startswith("cross_kernel_wide") and "128x512, 512 threads" in n
              and n.count(",") == 2]
     direct = _direct(lib, form)
-    assert len(plain) == 1 and len(direct) == 1, names
+    assert len(plain) == 1 and len(direct) == 2, names
+    assert "translation operand precomputed" in names[direct[1]]
     for prob in (c1[0], _ragged_problem()):
         projs = [lib.Projector(v) for v in prob.vols]
         try:
             got = []
-            for v in (plain[0], direct[0]):
+            for v in (plain[0], *direct):
                 lib.set_variant_override(v)
                 plan = lib.plan_for_problem(prob, projs, form)
                 t = lib.problem_tensors(prob)
@@ -508,7 +513,8 @@ def test_direct_kernel_is_bit_identical(c1, lib, torch_cuda, form):
                 got.append((t["diff2"].cpu().numpy(), t["best_index"].cpu().numpy(),
                             t["best_diff2"].cpu().numpy()))
                 plan.close()
-            for a, b in zip(got[0], got[1]):
-                assert np.array_equal(a, b), (form, plain, direct)
+            for other in got[1:]:
+                for a, b in zip(got[0], other):
+                    assert np.array_equal(a, b), (form, plain, direct)
         finally:
             lib.set_variant_override(-1)
