@@ -767,7 +767,8 @@ int rc_projector_create(int xdim, int ydim, int zdim, int inity, int initz, int 
     *out = nullptr;
     RC_REQUIRE(zdim != 1 && zdim != 0, RC_ERR_UNSUPPORTED,
                "2D references (zdim == 1) are not on the coarse 3D path");
-    RC_REQUIRE(xdim > 0 && ydim > 0 && zdim > 0, RC_ERR_INVALID, "bad volume dimensions");
+    RC_REQUIRE(xdim >= 2 && ydim > 0 && zdim > 0, RC_ERR_INVALID,
+               "bad volume dimensions (xdim must be at least 2)");
     RC_REQUIRE(padding_factor > 0.f && maxr > 0, RC_ERR_INVALID, "bad maxr or padding factor");
     RC_REQUIRE(host_real && host_imag, RC_ERR_INVALID, "NULL volume data");
     const size_t n = (size_t)xdim * ydim * zdim;

@@ -14,6 +14,9 @@ typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// Two adjacent float2 voxels (8-byte aligned, as the volume's float2 are).
+typedef float vox_pair __attribute__((ext_vector_type(4), aligned(8)));
+
 // Central-slice sample of the padded volume for image pixel (x, y) and the
 // orientation rows (e0, e1), (e3, e4), (e6, e7): the arithmetic of
 // AccProjectorKernel::project3Dmodel (acc_projectorkernel_impl.h:178-252)
@@ -39,14 +42,26 @@ __device__ __forceinline__ float2 project_sample(const ProjView &v, float e0, fl
     const float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
     const float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
     const int x0 = (int)fx0, y0 = (int)fy0, z0 = (int)fz0;
-    const int xa = clampi(x0, 0, v.mdlX - 1), xb = clampi(x0 + 1, 0, v.mdlX - 1);
+    // The two x neighbours are adjacent voxels: one 16-byte load per (y, z)
+    // corner pair, at xa (neighbours xa, xa + 1) or, where x is clamped at
+    // the last voxel (xa = xb = mdlX - 1), at mdlX - 2 with its second voxel
+    // used twice.  (mdlX >= 2 for every volume the library accepts.)
+    const int xa = clampi(x0, 0, v.mdlX - 1);
+    const int xl = min(xa, v.mdlX - 2);
+    const bool edge = xl != xa;
     const int ya = clampi(y0, 0, v.mdlY - 1) * v.mdlX, yb = clampi(y0 + 1, 0, v.mdlY - 1) * v.mdlX;
-    const size_t za = (size_t)clampi(z0, 0, v.mdlZ - 1) * v.mdlXY;
-    const size_t zb = (size_t)clampi(z0 + 1, 0, v.mdlZ - 1) * v.mdlXY;
-    const float2 d000 = v.vol[za + ya + xa], d001 = v.vol[za + ya + xb];
-    const float2 d010 = v.vol[za + yb + xa], d011 = v.vol[za + yb + xb];
-    const float2 d100 = v.vol[zb + ya + xa], d101 = v.vol[zb + ya + xb];
-    const float2 d110 = v.vol[zb + yb + xa], d111 = v.vol[zb + yb + xb];
+    const size_t za = (size_t)clampi(z0, 0, v.mdlZ - 1) * v.mdlXY + xl;
+    const size_t zb = (size_t)clampi(z0 + 1, 0, v.mdlZ - 1) * v.mdlXY + xl;
+    auto pair = [&](size_t i, float2 &lo, float2 &hi) {
+        const vox_pair w = *reinterpret_cast<const vox_pair *>(v.vol + i);
+        hi = make_float2(w.z, w.w);
+        lo = edge ? hi : make_float2(w.x, w.y);
+    };
+    float2 d000, d001, d010, d011, d100, d101, d110, d111;
+    pair(za + ya, d000, d001);
+    pair(za + yb, d010, d011);
+    pair(zb + ya, d100, d101);
+    pair(zb + yb, d110, d111);
     float2 r;
     {
         float dx00 = d000.x + (d001.x - d000.x) * fx;

@@ -35,9 +35,13 @@
 namespace rc {
 
 // ---------------------------------------------------------------------------
-// projection precompute in the wide encoding: block = 16 orientations x one
-// 16-pixel chunk
+// projection precompute in the wide encoding: block = 16 orientations x
+// kProjChunks 16-pixel chunks; each thread samples one orientation at one
+// pixel of every chunk, so its orientation rows are loaded once and the
+// gathers of its samples are in flight together
 // ---------------------------------------------------------------------------
+constexpr int kProjChunks = 2;
+
 template <int FORM, bool DIRECT>
 __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__restrict__ vols,
                                                            const float *__restrict__ bscale,
@@ -48,40 +52,65 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
                                                            float *__restrict__ pabs2)
 {
     constexpr int KB = WideEnc<FORM>::kBytes;
-    __shared__ __attribute__((aligned(16))) unsigned char s_out[16 * KB];
+    __shared__ __attribute__((aligned(16))) unsigned char s_out[kProjChunks][16 * KB];
     const int nkc = kpad / 16;
-    const int kc = blockIdx.x, cls = blockIdx.z;
+    const int cls = blockIdx.z;
     const int ol = threadIdx.x >> 4, pl = threadIdx.x & 15;
     const int o = blockIdx.y * 16 + ol;
-    const int p = kc * 16 + pl;
     const ProjView v = vols[cls];
-    float2 val = make_float2(0.f, 0.f);
-    if (o < O && p < npix) {
-        const float *e = eulers + 9 * (size_t)o;
-        const float2 xy = coords[p];
-        val = project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+    float2 val[kProjChunks];
+    float e[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (o < O) {
+        const float *eo = eulers + 9 * (size_t)o;
+        e[0] = eo[0];
+        e[1] = eo[1];
+        e[2] = eo[3];
+        e[3] = eo[4];
+        e[4] = eo[6];
+        e[5] = eo[7];
+    }
+#pragma unroll
+    for (int h = 0; h < kProjChunks; ++h) {
+        const int p = (blockIdx.x * kProjChunks + h) * 16 + pl;
+        val[h] = make_float2(0.f, 0.f);
+        if (o < O && p < npix) {
+            const float2 xy = coords[p];
+            val[h] = project_sample(v, e[0], e[1], e[2], e[3], e[4], e[5], xy.x, xy.y);
+        }
     }
     const float sc = (FORM == kFP16) ? bscale[cls] : 1.f;
-    wide_store_value<FORM>(s_out + ol * KB, pl, val.x, val.y, sc);
-    if (o < O) pabs2[((size_t)cls * O + o) * kpad + p] = val.x * val.x + val.y * val.y;
+#pragma unroll
+    for (int h = 0; h < kProjChunks; ++h) {
+        const int kc = blockIdx.x * kProjChunks + h;
+        wide_store_value<FORM>(s_out[h] + ol * KB, pl, val[h].x, val[h].y, sc);
+        if (o < O && kc < nkc)
+            pabs2[((size_t)cls * O + o) * kpad + kc * 16 + pl] =
+                val[h].x * val[h].x + val[h].y * val[h].y;
+    }
     __syncthreads();
     constexpr int units = 16 * KB / 16;
-    if constexpr (DIRECT) {
-        // the whole group, orientations past O included (zeros)
-        unsigned char *grp =
-            proj + (((size_t)cls * nkc + kc) * gridDim.y + blockIdx.y) * (16 * KB);
-        for (int u = threadIdx.x; u < units; u += 256) {
-            const int oll = u / (KB / 16), seg = u % (KB / 16);
-            *reinterpret_cast<uint4 *>(grp + direct_seg_offset<FORM>(oll, seg)) =
-                reinterpret_cast<const uint4 *>(s_out)[u];
-        }
-    } else {
-        for (int u = threadIdx.x; u < units; u += 256) {
-            const int oo = blockIdx.y * 16 + u / (KB / 16);
-            if (oo < O) {
-                const int seg = u % (KB / 16);
-                const uint4 d = reinterpret_cast<const uint4 *>(s_out)[u];
-                reinterpret_cast<uint4 *>(proj + (((size_t)cls * nkc + kc) * O + oo) * KB)[seg] = d;
+#pragma unroll
+    for (int h = 0; h < kProjChunks; ++h) {
+        const int kc = blockIdx.x * kProjChunks + h;
+        if (kc >= nkc) break;
+        if constexpr (DIRECT) {
+            // the whole group, orientations past O included (zeros)
+            unsigned char *grp =
+                proj + (((size_t)cls * nkc + kc) * gridDim.y + blockIdx.y) * (16 * KB);
+            for (int u = threadIdx.x; u < units; u += 256) {
+                const int oll = u / (KB / 16), seg = u % (KB / 16);
+                *reinterpret_cast<uint4 *>(grp + direct_seg_offset<FORM>(oll, seg)) =
+                    reinterpret_cast<const uint4 *>(s_out[h])[u];
+            }
+        } else {
+            for (int u = threadIdx.x; u < units; u += 256) {
+                const int oo = blockIdx.y * 16 + u / (KB / 16);
+                if (oo < O) {
+                    const int seg = u % (KB / 16);
+                    const uint4 d = reinterpret_cast<const uint4 *>(s_out[h])[u];
+                    reinterpret_cast<uint4 *>(proj + (((size_t)cls * nkc + kc) * O + oo) * KB)[seg] =
+                        d;
+                }
             }
         }
     }
@@ -99,7 +128,7 @@ hipError_t launch_project_wide(int form, const ProjView *vols, const float *bsca
                                int kpad, unsigned char *proj, float *pabs2, bool direct,
                                hipStream_t s)
 {
-    dim3 grid(kpad / 16, (O + 15) / 16, n_classes);
+    dim3 grid((kpad / 16 + kProjChunks - 1) / kProjChunks, (O + 15) / 16, n_classes);
 #define RC_PW(F, D)                                                                         \
     hipLaunchKernelGGL((project_wide_kernel<F, D>), grid, dim3(256), 0, s, vols, bscale, eulers, \
                        O, coords, npix, kpad, proj, pabs2)

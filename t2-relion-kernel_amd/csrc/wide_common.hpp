@@ -437,8 +437,20 @@ constexpr int EG = (FM >= 2 && FORM != kFP16) ? 2 : 1;   // fragment rows per lo
 // (the CC score's float64 division and square root among it) into the
 // instruction stream of the common form; every wave fetches the whole
 // epilogue once per tile, so its size is paid for on each tile.
+// Row partials: after the 16-lane reduction every lane of a row group holds
+// the row's (min, index); row j = 4 fa + v of lane group q is stored by lane
+// r16 = j % 16 from its slot j / 16, so that a wave stores its partials with
+// NS full-wave stores instead of one four-lane store per row.
+constexpr int NS = (4 * FM + 15) / 16;
 auto epilogue = [&](auto mode_c) {
     constexpr int MODE = decltype(mode_c)::value;
+    double mine_min[NS];
+    long long mine_idx[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        mine_min[s] = DBL_MAX;
+        mine_idx[s] = 0x7fffffffffffffffLL;
+    }
     int e_part[EG], e_b0[EG], e_T[EG];
     bool e_same[EG];
     double e_xn[EG];
@@ -483,34 +495,35 @@ auto epilogue = [&](auto mode_c) {
         int R0[EG], rT[EG][4], rt[EG][4];
         bool rok[EG][4];
         double best[EG][4];
-        int best_fb[EG];   // byte v: the column slice of row v's minimum, 0xff for none
+        int best_fb[EG][4];   // the column slice of row v's minimum, -1 for none
         static_for<0, EG>([&](auto e_c) {
             constexpr int e = decltype(e_c)::value;
             R0[e] = mtile * TM + (wm * FM + g * EG + e) * 16 + 4 * q;
             const int T = e_T[e] - e_b0[e];
-            best_fb[e] = -1;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 rok[e][v] = true;
                 rT[e][v] = T;
                 rt[e][v] = R0[e] + v - e_b0[e];
                 best[e][v] = DBL_MAX;
+                best_fb[e][v] = -1;
             }
         });
-        auto set_fb = [](int &w, int v, int fb) { w = (w & ~(0xff << (8 * v))) | (fb << (8 * v)); };
         // Common case: the lane's four rows are four consecutive
         // translations of one particle, 16 contiguous output bytes.
         // Orientation-major, so that the group's fragment rows of one
         // orientation -- adjacent runs of one output row -- are stored
         // back to back.
-        // output offset of (orientation o0 + rb0 + r16, the lane's first
+        // output address of (orientation o0 + rb0 + r16, the lane's first
         // row); column slice fb adds 16 fb orientations of T translations
-        size_t k00[EG];
+        float *p00[EG];
+        int fb_step[EG];
         static_for<0, EG>([&](auto e_c) {
             constexpr int e = decltype(e_c)::value;
             const int T = rT[e][0];
-            k00[e] = (size_t)C * O * e_b0[e] + (size_t)cls * O * T +
-                     (size_t)(o0 + rb0 + r16) * T + rt[e][0];
+            p00[e] = a.diff2 + ((size_t)C * O * e_b0[e] + (size_t)cls * O * T +
+                                (size_t)(o0 + rb0 + r16) * T + rt[e][0]);
+            fb_step[e] = 16 * T;
         });
 #pragma unroll
         for (int fb = 0; fb < FN; ++fb) {
@@ -519,10 +532,11 @@ auto epilogue = [&](auto mode_c) {
                 constexpr int e = decltype(e_c)::value;
                 constexpr int fa = g * EG + e;
                 if (o < O && e_same[e]) {
-                    const size_t k0 = k00[e] + (size_t)(fb * 16 * rT[e][0]);
+                    float *const dst = p00[e] + fb * fb_step[e];
                     float init[4] = {0.f, 0.f, 0.f, 0.f};
                     if constexpr (MODE == kEpiInit) {
-                        const f4u x = *reinterpret_cast<const f4u *>(a.diff2_init + k0);
+                        const f4u x = *reinterpret_cast<const f4u *>(
+                            a.diff2_init + (dst - a.diff2));
                         init[0] = x.x;
                         init[1] = x.y;
                         init[2] = x.z;
@@ -536,12 +550,11 @@ auto epilogue = [&](auto mode_c) {
                             acc[fb][fa][v] * (FORM == kFP16 ? e_inv[FORM == kFP16 ? e : 0] : 1.f);
                         const double d = score(cval, pn, e_xn[e], init[v]);
                         out[v] = (float)d;
-                        if (d < best[e][v]) {
-                            best[e][v] = d;
-                            set_fb(best_fb[e], v, fb);
-                        }
+                        const bool lt = d < best[e][v];
+                        best[e][v] = lt ? d : best[e][v];
+                        best_fb[e][v] = lt ? fb : best_fb[e][v];
                     }
-                    *reinterpret_cast<f4u *>(a.diff2 + k0) = f4u{out[0], out[1], out[2], out[3]};
+                    *reinterpret_cast<f4u *>(dst) = f4u{out[0], out[1], out[2], out[3]};
                 }
             });
         }
@@ -587,7 +600,7 @@ auto epilogue = [&](auto mode_c) {
                         a.diff2[rb[v] + (size_t)o * rT[e][v] + rt[e][v]] = (float)d;
                         if (d < best[e][v]) {
                             best[e][v] = d;
-                            set_fb(best_fb[e], v, fb);
+                            best_fb[e][v] = fb;
                         }
                     }
                 }
@@ -609,22 +622,33 @@ auto epilogue = [&](auto mode_c) {
                     const double ob = ror16<(8 >> decltype(st_c)::value)>(bv);
                     bv = ob < bv ? ob : bv;
                 });
-                const int bfb = (best_fb[e] >> (8 * v)) & 0xff;
-                int bo = bfb != 0xff && best[e][v] == bv ? rb0 + bfb * 16 + r16 : 0x7fffffff;
+                const int bfb = best_fb[e][v];
+                int bo = bfb >= 0 && best[e][v] == bv ? rb0 + bfb * 16 + r16 : 0x7fffffff;
                 static_for<0, 4>([&](auto st_c) {
                     bo = min(bo, ror16<(8 >> decltype(st_c)::value)>(bo));
                 });
-                if (r16 == 0 && rok[e][v]) {
-                    const size_t u = ((size_t)(R0[e] + v) * n_ntiles + ntile) * WN + wn;
-                    a.part_min[u] = bv;
-                    a.part_idx[u] = bo == 0x7fffffff
-                                        ? 0x7fffffffffffffffLL
-                                        : (long long)(cls * O + o0 + bo) * rT[e][v] + rt[e][v];
-                }
+                constexpr int j0 = 4 * (g * EG + e);   // row j0 + v of the lane group
+                const long long idx = bo == 0x7fffffff
+                                          ? 0x7fffffffffffffffLL
+                                          : (long long)(cls * O + o0 + bo) * rT[e][v] + rt[e][v];
+                const bool mine = r16 == ((j0 + v) & 15);
+                mine_min[(j0 + v) >> 4] = mine ? bv : mine_min[(j0 + v) >> 4];
+                mine_idx[(j0 + v) >> 4] = mine ? idx : mine_idx[(j0 + v) >> 4];
             }
         });
     };
     static_for<0, FM / EG>(epilogue_group);
+    if (!a.part_min) return;
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) {
+        const int j = sl * 16 + r16;   // row 4 fa + v of the lane group
+        const int R = mtile * TM + (wm * FM + (j >> 2)) * 16 + 4 * q + (j & 3);
+        if (j < 4 * FM && R < a.n_rows) {
+            const size_t u = ((size_t)R * n_ntiles + ntile) * WN + wn;
+            a.part_min[u] = mine_min[sl];
+            a.part_idx[u] = mine_idx[sl];
+        }
+    }
 };
 if (a.cc)
     epilogue(std::integral_constant<int, kEpiCC>{});
