@@ -373,7 +373,7 @@ def downstream_bench(lib, torch, prob, t, steps, cpu_seconds):
         workload=f"{P} particles x {samples // P} samples, adaptive fraction 0.999")
     # ---- fine pass of particle 0
     g = prob.geom
-    order, _ = synth.filter_order(g)
+    order, n_inside = synth.filter_order(g)
     inv = np.empty_like(order)
     inv[order] = np.arange(order.size)
     pb = prob.particles
@@ -421,6 +421,14 @@ def downstream_bench(lib, torch, prob, t, steps, cpu_seconds):
                 pixel_evals_per_s=fevals * g.image_size / (fms * 1e-3),
                 workload=f"1 particle, {len(job_idx)} jobs x 4 translations = {fevals} fine "
                          f"evaluations over {g.image_size} pixels")
+    # Each job projects its own orientation at every in-radius pixel: 8 voxels
+    # (four 16-byte pair loads) per sample, from L2 / the last-level cache.
+    gather = len(job_idx) * int(n_inside) * 64
+    fine["roofline"] = dict(
+        bound="gather", achieved=gather / (fms * 1e-3) / 1e9, unit="GB/s",
+        bytes_per_launch=gather,
+        note="trilinear gathers of the per-job projections (jobs x in-radius pixels x 64 B); "
+             "no published peak applies to scattered 16-byte cache reads, so no fraction")
     if cpu_seconds > 0:
         from oracle import oracle
         take = len(job_idx)        # the whole job list: well under a minute on the host

@@ -170,25 +170,51 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
 // ---------------------------------------------------------------------------
 // |P|^2 norms: pnorm[i][n] = sum_p half_corr[i][p] * pabs2[n][p]
 // ---------------------------------------------------------------------------
-// A small GEMM on the matrix cores (v_mfma_f32_16x16x4_f32): block tile
-// 64 particles x 64 columns, 4 waves of 32 x 32, k staged through LDS in
-// 64-pixel chunks (rows padded to 68 floats: 16-byte stores, conflict-free
-// column reads); the next chunk's global loads are issued before the
-// current chunk's MFMAs.  A[i][k] = half_corr, B[k][n] = |P|^2.  kpad is a
-// multiple of 32; a last half chunk is zero-filled.
-// Summation: float32 products summed in float32 within a 64-pixel chunk
-// only; the chunk sums are added in float64 (a cascaded sum), so the
-// rounding error grows with the chunk length (64), not with the pixel count.
-constexpr int kPnK = 64;
-constexpr int kPnRow = kPnK + 4;
+// A small GEMM on the BF16 matrix cores with float32 operands carried
+// exactly: each float32 value is split into three bfloat16 pieces,
+// v = v1 + v2 + v3 (8 significant bits each, RNE; the remainders are exact
+// in float32, so the split loses nothing), and the product a b is summed as
+// the six piece products with i + j <= 4: a3 b1 + a1 b3 + a2 b2 + a2 b1 +
+// a1 b2 + a1 b1 (smallest first; the omitted a2 b3, a3 b2, a3 b3 are below
+// 2^-25 of |a b|).  Every piece product is exact and accumulated in float32
+// by v_mfma_f32_16x16x32_bf16: six 16-cycle MFMAs per 16 x 16 x 32 block
+// instead of eight 32-cycle FP32 ones.
+// Block tile 64 particles x 64 columns, 4 waves of 32 x 32 (2 x 2 fragments);
+// k staged through LDS in 32-pixel chunks as [piece][row][32 bf16] (rows
+// padded to 80 bytes), the next chunk's global loads issued before the
+// current chunk's MFMAs.  A[i][k] = half_corr, B[k][n] = |P|^2; kpad is a
+// multiple of 32.
+// Summation: float32 within a 32-pixel chunk, chunk sums added in float64
+// (a cascaded sum), so the rounding error grows with the chunk length, not
+// with the pixel count.
+constexpr int kPnK = 32;
+constexpr int kPnRowB = kPnK * 2 + 16;   // bytes per staged row (64 + 16 pad)
+
+typedef float pn_f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 pn_bf16x2 __attribute__((ext_vector_type(2)));
+
+// the three bfloat16 pieces of two float32 values, as three packed words
+__device__ __forceinline__ void split3_bf16(float x, float y, uint32_t &w1, uint32_t &w2,
+                                            uint32_t &w3)
+{
+    const pn_f32x2 v = {x, y};
+    const pn_bf16x2 h1 = __builtin_convertvector(v, pn_bf16x2);
+    const pn_f32x2 r1 = v - __builtin_convertvector(h1, pn_f32x2);
+    const pn_bf16x2 h2 = __builtin_convertvector(r1, pn_bf16x2);
+    const pn_f32x2 r2 = r1 - __builtin_convertvector(h2, pn_f32x2);
+    const pn_bf16x2 h3 = __builtin_convertvector(r2, pn_bf16x2);
+    w1 = __builtin_bit_cast(uint32_t, h1);
+    w2 = __builtin_bit_cast(uint32_t, h2);
+    w3 = __builtin_bit_cast(uint32_t, h3);
+}
 
 __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
                                                          const float *__restrict__ hc,
                                                          const float *__restrict__ pabs2,
                                                          float *__restrict__ pnorm)
 {
-    __shared__ __attribute__((aligned(16))) float sA[64][kPnRow];
-    __shared__ __attribute__((aligned(16))) float sB[64][kPnRow];
+    __shared__ __attribute__((aligned(16))) unsigned char sA[3][64 * kPnRowB];
+    __shared__ __attribute__((aligned(16))) unsigned char sB[3][64 * kPnRowB];
     // XCD-aware order: block b runs on XCD b % 8; each XCD takes a contiguous
     // range of tiles in which the particle tiles of one column tile are
     // adjacent, so that column tile's |P|^2 rows come from its L2
@@ -202,22 +228,33 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
     const int r16 = lane & 15, q = lane >> 4;
-    // this thread's share of a chunk: row lr, k offsets lk .. lk + 15
-    const int lr = tid >> 2, lk = (tid & 3) * 16;
+    // this thread's share of a chunk: row lr, k offsets lk .. lk + 7
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
     const bool a_ok = i0 + lr < P, b_ok = n0 + lr < N;
     const float *ap = hc + (size_t)(a_ok ? i0 + lr : 0) * kpad + lk;
     const float *bp = pabs2 + (size_t)(b_ok ? n0 + lr : 0) * kpad + lk;
     const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 ra[4], rb[4];
+    float4 ra[2], rb[2];
     auto fetch = [&](int k0) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const bool in = k0 + lk + 4 * u < kpad;
+        for (int u = 0; u < 2; ++u) {
             ra[u] = zero;
             rb[u] = zero;
-            if (a_ok && in) ra[u] = *reinterpret_cast<const float4 *>(ap + k0 + 4 * u);
-            if (b_ok && in) rb[u] = *reinterpret_cast<const float4 *>(bp + k0 + 4 * u);
+            if (a_ok) ra[u] = *reinterpret_cast<const float4 *>(ap + k0 + 4 * u);
+            if (b_ok) rb[u] = *reinterpret_cast<const float4 *>(bp + k0 + 4 * u);
         }
+    };
+    // the 8 values of a thread's share, as the three pieces' 16-byte segments
+    auto stage = [&](const float4 (&r)[2], unsigned char (*dst)[64 * kPnRowB]) {
+        uint32_t w[3][4];
+        split3_bf16(r[0].x, r[0].y, w[0][0], w[1][0], w[2][0]);
+        split3_bf16(r[0].z, r[0].w, w[0][1], w[1][1], w[2][1]);
+        split3_bf16(r[1].x, r[1].y, w[0][2], w[1][2], w[2][2]);
+        split3_bf16(r[1].z, r[1].w, w[0][3], w[1][3], w[2][3]);
+#pragma unroll
+        for (int pc = 0; pc < 3; ++pc)
+            *reinterpret_cast<uint4 *>(&dst[pc][lr * kPnRowB + lk * 2]) =
+                make_uint4(w[pc][0], w[pc][1], w[pc][2], w[pc][3]);
     };
     double dacc[2][2][4];
 #pragma unroll
@@ -228,29 +265,37 @@ __global__ __launch_bounds__(256) void pnorm_mfma_kernel(int P, int N, int kpad,
             for (int v = 0; v < 4; ++v) dacc[x][y][v] = 0.0;
     fetch(0);
     for (int k0 = 0; k0 < kpad; k0 += kPnK) {
+        __syncthreads();
+        stage(ra, sA);
+        stage(rb, sB);
+        __syncthreads();
+        if (k0 + kPnK < kpad) fetch(k0 + kPnK);
+        bf16x8 fa[2][3], fb[2][3];
+#pragma unroll
+        for (int pc = 0; pc < 3; ++pc)
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                fa[x][pc] = *reinterpret_cast<const bf16x8 *>(
+                    &sA[pc][(wr + 16 * x + r16) * kPnRowB + q * 16]);
+                fb[x][pc] = *reinterpret_cast<const bf16x8 *>(
+                    &sB[pc][(wc + 16 * x + r16) * kPnRowB + q * 16]);
+            }
         floatx4 acc[2][2];
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
             for (int y = 0; y < 2; ++y) acc[x][y] = floatx4{0.f, 0.f, 0.f, 0.f};
-        __syncthreads();
+        // piece pairs (i, j), i + j <= 4, smallest first
+        constexpr int kPi[6] = {2, 0, 1, 1, 0, 0};
+        constexpr int kPj[6] = {0, 2, 1, 0, 1, 0};
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            *reinterpret_cast<float4 *>(&sA[lr][lk + 4 * u]) = ra[u];
-            *reinterpret_cast<float4 *>(&sB[lr][lk + 4 * u]) = rb[u];
-        }
-        __syncthreads();
-        if (k0 + kPnK < kpad) fetch(k0 + kPnK);
+        for (int t = 0; t < 6; ++t)
 #pragma unroll
-        for (int s = 0; s < kPnK / 4; ++s) {
-            const int k = 4 * s + q;
-            const float a0 = sA[wr + r16][k], a1 = sA[wr + 16 + r16][k];
-            const float b0 = sB[wc + r16][k], b1 = sB[wc + 16 + r16][k];
-            acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b0, acc[0][0], 0, 0, 0);
-            acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b1, acc[0][1], 0, 0, 0);
-            acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b0, acc[1][0], 0, 0, 0);
-            acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b1, acc[1][1], 0, 0, 0);
-        }
+            for (int x = 0; x < 2; ++x)
+#pragma unroll
+                for (int y = 0; y < 2; ++y)
+                    acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        fa[x][kPi[t]], fb[y][kPj[t]], acc[x][y], 0, 0, 0);
 #pragma unroll
         for (int x = 0; x < 2; ++x)
 #pragma unroll
