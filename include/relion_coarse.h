@@ -132,9 +132,9 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
  * reference; for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops).  Same arguments, semantics, kernels
  * and results (bit-identical) as rc_run_diff2_kernel_coarse; its scratch is
- * a 40 MiB device array inside the library (one per GPU, held until the
+ * a 32 MiB device array inside the library (one per GPU, held until the
  * process ends), so shapes whose workspace (rc_coarse_workspace_bytes with
- * the FP32 form) exceeds 40 MiB are rejected with RC_ERR_LIMIT before any
+ * the FP32 form) exceeds 32 MiB are rejected with RC_ERR_LIMIT before any
  * launch.  Calls that overlap on one GPU -- any two streams, or host threads
  * -- share that array and race: serialise them. */
 int rc_run_diff2_kernel_coarse_noalloc(const rc_projector_kernel *projector,

@@ -100,12 +100,16 @@ struct SingleShape {
     int ksplit, kc_per_split;
 };
 
+// Fused tile of the single-image calls, by measurement at 21 x 576 x 544 and
+// 113 x 4608 x 2112 px: 64 x 128 for the FP32 form (the drop-in's, whose
+// projections usually come from the cache, 1.4x faster than 128 x 128 at
+// 113 translations) and for up to 64 translations; 128 x 128 for the split
+// forms beyond, where the fused gather of each row tile's projections is
+// what costs (BF16x3: 287 against 386 us).
 static int pick_fused_variant(int form, int T)
 {
-    (void)form;
-    if (T <= 32) return 2;   // 32 x 256
-    if (T <= 64) return 1;   // 64 x 128
-    return 0;                // 128 x 128
+    if (form == RC_FORM_FP32 || T <= 64) return 1;   // 64 x 128
+    return 0;                                        // 128 x 128
 }
 
 static SingleShape single_shape(int T, int O, int npix, int form)
@@ -132,7 +136,8 @@ static SingleShape single_shape(int T, int O, int npix, int form)
 // Device buffers of the drop-in's projection cache (see ScratchEntry).
 struct ProjCacheView {
     float *eulers;   // copy of the eulers the projections were made from
-    int *flag;       // set by the check kernel when they must be made again
+    unsigned *flag;  // set to gen by the check kernel when they must be made again
+    unsigned gen;    // this call's generation (one more than the last call's)
     float2 *P;       // [O][kpad]
 };
 
@@ -233,9 +238,10 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
     a.part_pn = w.part_pn;
     a.pcache = nullptr;
     if (cache) {
-        RC_HIP(launch_cache_check(eulers, cache->eulers, 9 * O, cache_force, cache->flag, stream));
-        RC_HIP(launch_cache_fill(cache->flag, v, eulers, O, w.coords, npix, kpad, cache->P,
-                                 stream));
+        RC_HIP(launch_cache_check(eulers, cache->eulers, 9 * O, cache_force, cache->gen,
+                                  cache->flag, stream));
+        RC_HIP(launch_cache_fill(cache->flag, cache->gen, v, eulers, O, w.coords, npix, kpad,
+                                 cache->P, stream));
         a.pcache = cache->P;
     }
     RC_HIP(launch_cross(form, true, vi, a, stream));
@@ -303,6 +309,7 @@ struct ScratchEntry {
     size_t cache_bytes = 0;
     ProjCacheKey key;
     bool cache_valid = false;
+    unsigned cache_gen = 0;   // generation of the last call (see ProjCacheView)
 };
 // Largest projection cache per stream (float2 per orientation and pixel);
 // larger searches gather their projections every call.
@@ -371,7 +378,8 @@ static int proj_cache_for(ScratchEntry *e, hipStream_t stream, const ProjCacheKe
     }
     unsigned char *b = static_cast<unsigned char *>(e->cache);
     view->eulers = reinterpret_cast<float *>(b);
-    view->flag = reinterpret_cast<int *>(b + eb);
+    view->flag = reinterpret_cast<unsigned *>(b + eb);
+    view->gen = ++e->cache_gen;
     view->P = reinterpret_cast<float2 *>(b + eb + 256);
     *force = !(e->cache_valid && e->key == key);
     e->key = key;

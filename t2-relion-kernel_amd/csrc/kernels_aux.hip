@@ -3,6 +3,7 @@
 // float32 kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 
 #include "device_common.hpp"
@@ -360,52 +361,75 @@ __global__ __launch_bounds__(256) void splitk_assemble_kernel(int O, int T, int 
 
 // One workgroup: compares the n floats bit for bit (a reused buffer with
 // new contents is a new key), then refreshes the copy if anything differs.
+// The check runs over many blocks, each comparing a 1024-float slice of the
+// eulers with the cached copy; a block whose slice differs (or every block,
+// when forced) refreshes its slice and writes this call's generation into
+// the flag.  The flag is never cleared: the fill compares it with the same
+// generation, so a call whose eulers all match leaves an older value there
+// and the fill does nothing.
+constexpr int kCheckSlice = 1024;
+
 __global__ __launch_bounds__(256) void cache_check_kernel(const float *__restrict__ eulers,
                                                           float *__restrict__ copy, int n,
-                                                          int force, int *flag)
+                                                          int force, unsigned gen,
+                                                          unsigned *flag)
 {
     __shared__ int s_diff;
     if (threadIdx.x == 0) s_diff = force;
     __syncthreads();
+    const int i0 = blockIdx.x * kCheckSlice, i1 = min(n, i0 + kCheckSlice);
     int diff = 0;
-    for (int i = threadIdx.x; i < n; i += 256)
+    for (int i = i0 + threadIdx.x; i < i1; i += 256)
         diff |= __float_as_uint(eulers[i]) != __float_as_uint(copy[i]);
     if (diff) s_diff = 1;
     __syncthreads();
-    const int d = s_diff;
-    if (d)
-        for (int i = threadIdx.x; i < n; i += 256) copy[i] = eulers[i];
-    if (threadIdx.x == 0) *flag = d;
+    if (s_diff) {
+        for (int i = i0 + threadIdx.x; i < i1; i += 256) copy[i] = eulers[i];
+        if (threadIdx.x == 0) *flag = gen;
+    }
 }
 
-hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, int *flag,
-                              hipStream_t s)
+hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, unsigned gen,
+                              unsigned *flag, hipStream_t s)
 {
-    hipLaunchKernelGGL(cache_check_kernel, dim3(1), dim3(256), 0, s, eulers, copy, n, force, flag);
+    const int blocks = std::max(1, (n + kCheckSlice - 1) / kCheckSlice);
+    hipLaunchKernelGGL(cache_check_kernel, dim3(blocks), dim3(256), 0, s, eulers, copy, n, force,
+                       gen, flag);
     return hipGetLastError();
 }
 
-// block = 256 pixels of one orientation; the same sample as the fused
-// kernels' gather (rows e0, e1, e3, e4, e6, e7, pixel coordinates from prep)
-__global__ __launch_bounds__(256) void cache_fill_kernel(const int *flag, const ProjView v,
+// A grid of at most kFillBlocks blocks striding over (orientation, pixel):
+// when the flag does not hold this call's generation every block returns at
+// once.  The sample is the fused kernels' gather (rows e0, e1, e3, e4, e6,
+// e7, pixel coordinates from prep).
+constexpr int kFillBlocks = 2048;
+
+__global__ __launch_bounds__(256) void cache_fill_kernel(const unsigned *flag, unsigned gen,
+                                                         const ProjView v,
                                                          const float *__restrict__ eulers,
                                                          const float2 *__restrict__ coords,
-                                                         int npix, int kpad, float2 *P)
+                                                         int O, int npix, int kpad, float2 *P)
 {
-    if (*flag == 0) return;
-    const int o = blockIdx.y;
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= npix) return;
-    const float *e = eulers + 9 * (size_t)o;
-    const float2 xy = coords[p];
-    P[(size_t)o * kpad + p] = project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+    if (*flag != gen) return;
+    const size_t total = (size_t)O * npix;
+    for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < total;
+         k += (size_t)gridDim.x * 256) {
+        const int o = (int)(k / npix), p = (int)(k - (size_t)o * npix);
+        const float *e = eulers + 9 * (size_t)o;
+        const float2 xy = coords[p];
+        P[(size_t)o * kpad + p] =
+            project_sample(v, e[0], e[1], e[3], e[4], e[6], e[7], xy.x, xy.y);
+    }
 }
 
-hipError_t launch_cache_fill(const int *flag, const ProjView &v, const float *eulers, int O,
-                             const float2 *coords, int npix, int kpad, float2 *P, hipStream_t s)
+hipError_t launch_cache_fill(const unsigned *flag, unsigned gen, const ProjView &v,
+                             const float *eulers, int O, const float2 *coords, int npix, int kpad,
+                             float2 *P, hipStream_t s)
 {
-    hipLaunchKernelGGL(cache_fill_kernel, dim3((npix + 255) / 256, O), dim3(256), 0, s, flag, v,
-                       eulers, coords, npix, kpad, P);
+    const size_t total = (size_t)O * npix;
+    const int blocks = (int)std::min<size_t>(kFillBlocks, (total + 255) / 256);
+    hipLaunchKernelGGL(cache_fill_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, s, flag, gen, v,
+                       eulers, coords, O, npix, kpad, P);
     return hipGetLastError();
 }
 

@@ -11,11 +11,11 @@
 
 namespace rc {
 
-// 40 MiB: image staging plus the split-K partial sums, which the slice
-// rule caps at 32 MiB (a 113-translation, 4608-orientation, 1636-pixel
-// particle needs 27.4 MB in all).  Each GPU the library runs on holds one
-// copy until the process ends.
-constexpr size_t kScratchBytes = 40ull << 20;
+// 32 MiB: image staging (20 bytes per pixel) plus the split-K partial sums,
+// which the slice rule caps at 32 MiB (a 113-translation, 4608-orientation,
+// 2112-pixel particle needs 16.9 MB in all).  Each GPU the library runs on
+// holds one copy until the process ends.
+constexpr size_t kScratchBytes = 32ull << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 // Device address of `scratch` on each device, resolved once per device with

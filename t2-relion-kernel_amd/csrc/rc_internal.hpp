@@ -191,14 +191,17 @@ hipError_t launch_pnorm(int n_particles, int n_cols, int kpad, const float *half
 
 // Split-K assembly of one image: out[o*T+t] = score from the fixed-order
 // float64 sums over the S slices of part_c and part_pn (see CrossArgs).
-// Drop-in projection cache.  cache_check: flag = force, or the eulers
-// differ bit for bit from the cached copy; when set, the copy is refreshed.
-// cache_fill: when the flag is set, P[o][p] = the projection of orientation o
-// at pixel p (p < npix; the fused gather's arithmetic), else nothing.
-hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, int *flag,
-                              hipStream_t s);
-hipError_t launch_cache_fill(const int *flag, const ProjView &v, const float *eulers, int O,
-                             const float2 *coords, int npix, int kpad, float2 *P, hipStream_t s);
+// Drop-in projection cache.  cache_check: when forced, or where the eulers
+// differ bit for bit from the cached copy, the copy is refreshed and the
+// flag set to gen (the call's generation; the flag is never cleared).
+// cache_fill: when the flag holds gen, P[o][p] = the projection of
+// orientation o at pixel p (p < npix; the fused gather's arithmetic), else
+// nothing.
+hipError_t launch_cache_check(const float *eulers, float *copy, int n, int force, unsigned gen,
+                              unsigned *flag, hipStream_t s);
+hipError_t launch_cache_fill(const unsigned *flag, unsigned gen, const ProjView &v,
+                             const float *eulers, int O, const float2 *coords, int npix, int kpad,
+                             float2 *P, hipStream_t s);
 hipError_t launch_splitk_assemble(int O, int T, int S, const float *part_c, const float *part_pn,
                                   const double *xnorm, const float *init, float *out, int cc,
                                   hipStream_t s);
