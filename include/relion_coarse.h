@@ -97,11 +97,18 @@ int rc_projector_make_kernel(const rc_projector *p, int imgX, int imgY, int imgZ
  * sqrt(sum corr |P|^2) instead (diff2.cuh:691-809, overwriting, with that
  * kernel's pixel-coordinate rule).  data_is_3D != 0 -> RC_ERR_UNSUPPORTED.
  *
+ * Pixels: the search runs over the pixels that a rotation can bring inside
+ * the projector's mask, (x^2 + y^2) pad^2 <= (maxR2_padded + 1)(1 + 2^-12)
+ * in float32 (the coordinates of the pixel rule above), moved first; every
+ * other pixel projects to 0 for every rotation matrix, so it adds only
+ * corr/2 |X|^2 (diff2) or nothing (CC), which the call adds directly.  This
+ * holds for orthonormal eulers (rotations), as RELION's are.
+ *
  * Work split: the pixel axis is cut into slices (rc_coarse_split) so that
  * one image fills the device; each slice writes float32 partial sums and a
  * second kernel adds the slices in a fixed order in float64.  The slice count
- * depends on (translation_num, orientation_num, image_size) only, so a
- * shape always gives bit-identical results.
+ * depends on the shape and the image geometry only, so a shape always gives
+ * bit-identical results.
  *
  * Scratch: the reference signature has no workspace, so the call keeps one
  * scratch buffer per (device, stream) -- calls on different streams (e.g.

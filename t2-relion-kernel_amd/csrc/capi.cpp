@@ -174,7 +174,8 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                       const float *imag, const rc_projector_kernel *pk, const float *corr,
                       const float *init, float *dest, hipStream_t stream, const float *coor_xy,
                       void *ws, size_t ws_bytes, bool cc = false,
-                      const ProjCacheView *cache = nullptr, int cache_force = 0)
+                      const ProjCacheView *cache = nullptr, int cache_force = 0,
+                      bool partition = false)
 {
     if (int st = check_kernel_view(pk)) return st;
     RC_REQUIRE(T > 0 && O > 0 && npix > 0, RC_ERR_INVALID,
@@ -195,8 +196,19 @@ static int run_single(int T, int O, int npix, int full_npix, int form, const flo
                "workspace of %zu bytes is smaller than the %zu bytes needed", ws_bytes, w.bytes);
     const int kpad = (int)align_up((size_t)npix, kChunk);
     ProjView v = view_of(*pk);
-    RC_HIP(launch_prep(1, npix, full_npix, kpad, real, imag, corr, coor_xy, pk->imgX, pk->imgY,
-                       pk->maxR, w.xw, w.coords, w.xnorm, w.half_corr, w.ascale, stream, cc));
+    if (partition) {
+        // npix = the pixels a rotation can bring inside the mask, of the
+        // full_npix of the unfiltered image
+        float pad2 = 0.f, thr = 0.f;
+        RC_REQUIRE(dropin_pixels(*pk, full_npix, cc, &pad2, &thr) == npix, RC_ERR_INVALID,
+                   "partitioned pixel count %d does not match the geometry", npix);
+        RC_HIP(launch_prep_partition(full_npix, kpad, real, imag, corr, pk->imgX, pk->imgY,
+                                     pk->maxR, pad2, thr, w.xw, w.coords, w.xnorm, w.half_corr,
+                                     w.ascale, stream, cc));
+    } else {
+        RC_HIP(launch_prep(1, npix, full_npix, kpad, real, imag, corr, coor_xy, pk->imgX, pk->imgY,
+                           pk->maxR, w.xw, w.coords, w.xnorm, w.half_corr, w.ascale, stream, cc));
+    }
     if (form == RC_FORM_FP32_DIRECT) {
         RC_REQUIRE(!cc, RC_ERR_UNSUPPORTED, "the direct form has no cross-correlation mode");
         RC_REQUIRE(T <= 1024, RC_ERR_LIMIT, "direct form supports at most 1024 translations");
@@ -268,10 +280,54 @@ int single_split(int T, int O, int npix, int form, int *ksplit, int *kc_per_spli
 int run_single_ws(int T, int O, int npix, int full_npix, int form, const float *eulers,
                   const float *trans_x, const float *trans_y, const float *real, const float *imag,
                   const rc_projector_kernel *pk, const float *corr, const float *init, float *dest,
-                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc)
+                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc,
+                  bool partition)
 {
     return run_single(T, O, npix, full_npix, form, eulers, trans_x, trans_y, real, imag, pk, corr,
-                      init, dest, stream, coor_xy, ws, ws_bytes, cc);
+                      init, dest, stream, coor_xy, ws, ws_bytes, cc, nullptr, 0, partition);
+}
+
+int dropin_pixels(const rc_projector_kernel &k, int image_size, bool cc, float *pad2,
+                  float *thr)
+{
+    const float p2 = k.padding_factor * k.padding_factor;
+    const float t = ((float)k.maxR2_padded + 1.0f) * (1.0f + 1.0f / 4096.0f);
+    *pad2 = p2;
+    *thr = t;
+    // counted once per geometry (an image of 2^20 pixels takes a millisecond)
+    struct Key {
+        int imgX, imgY, maxR, n, cc;
+        float p2, t;
+        bool operator<(const Key &o) const
+        {
+            return std::tie(imgX, imgY, maxR, n, cc, p2, t) <
+                   std::tie(o.imgX, o.imgY, o.maxR, o.n, o.cc, o.p2, o.t);
+        }
+    };
+    static std::mutex mu;
+    static std::map<Key, int> counts;
+    const Key key{k.imgX, k.imgY, k.maxR, image_size, cc ? 1 : 0, p2, t};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = counts.find(key);
+        if (it != counts.end()) return it->second;
+    }
+    int n = 0;
+    for (int p = 0; p < image_size; ++p) {
+        int x = p % k.imgX, y = p / k.imgX;
+        if (y > k.maxR) {
+            if (!cc || y >= k.imgY - k.maxR)
+                y -= k.imgY;
+            else
+                x = k.maxR;
+        }
+        const volatile float r2 = (float)(x * x + y * y) * p2;   // one float multiply, as on the device
+        n += r2 <= t;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (counts.size() > 256) counts.clear();
+    counts[key] = n;
+    return n;
 }
 
 // Scratch of rc_run_diff2_kernel_coarse, one buffer per (device, stream):
@@ -292,12 +348,12 @@ struct ProjCacheKey {
     const rc_projector *proj = nullptr;
     unsigned long long generation = 0;
     const float *eulers = nullptr;
-    int O = 0, npix = 0, imgX = 0, imgY = 0, maxR = 0, maxR2 = 0;
+    int O = 0, npix = 0, full = 0, imgX = 0, imgY = 0, maxR = 0, maxR2 = 0;
     bool operator==(const ProjCacheKey &o) const
     {
         return proj == o.proj && generation == o.generation && eulers == o.eulers && O == o.O &&
-               npix == o.npix && imgX == o.imgX && imgY == o.imgY && maxR == o.maxR &&
-               maxR2 == o.maxR2;
+               npix == o.npix && full == o.full && imgX == o.imgX && imgY == o.imgY &&
+               maxR == o.maxR && maxR2 == o.maxR2;
     }
 };
 struct ScratchEntry {
@@ -951,13 +1007,19 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
     if (int st = check_drop_in(projector, orientation_num, translation_num, image_size,
                                data_is_3D))
         return st;
-    const size_t need = single_workspace_bytes((int)translation_num, (int)orientation_num,
-                                               (int)image_size, RC_FORM_FP32);
+    // the search runs over the pixels a rotation can bring inside the mask
+    // (the others only add corr/2 |X|^2); all of them when none can
+    float pad2, thr;
+    const int n_in = dropin_pixels(*projector, (int)image_size, do_CC != 0, &pad2, &thr);
+    const bool part = n_in > 0;
+    const int npix = part ? n_in : (int)image_size;
+    const size_t need = single_workspace_bytes((int)translation_num, (int)orientation_num, npix,
+                                               RC_FORM_FP32);
     std::unique_lock<std::mutex> held;
     void *ws = nullptr;
     ScratchEntry *entry = nullptr;
     if (int st = scratch_for((hipStream_t)stream, need, held, &ws, &entry)) return st;
-    const int kpad = (int)align_up((size_t)image_size, kChunk);
+    const int kpad = (int)align_up((size_t)npix, kChunk);
     ProjCacheView cache{};
     int force = 0;
     const bool use_cache = (size_t)orientation_num * kpad * sizeof(float2) <= kProjCacheMaxBytes &&
@@ -968,7 +1030,8 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
         key.generation = projector->proj->generation;
         key.eulers = d_eulers;
         key.O = (int)orientation_num;
-        key.npix = (int)image_size;
+        key.npix = npix;
+        key.full = (int)image_size;
         key.imgX = projector->imgX;
         key.imgY = projector->imgY;
         key.maxR = projector->maxR;
@@ -976,11 +1039,10 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector, const float
         if (int st = proj_cache_for(entry, (hipStream_t)stream, key, kpad, &cache, &force))
             return st;
     }
-    return run_single((int)translation_num, (int)orientation_num, (int)image_size,
-                      (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real,
-                      d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
-                      (hipStream_t)stream, nullptr, ws, need, do_CC != 0,
-                      use_cache ? &cache : nullptr, force);
+    return run_single((int)translation_num, (int)orientation_num, npix, (int)image_size,
+                      RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real, d_fimg_imag,
+                      projector, d_corr_img, d_diff2s, d_diff2s, (hipStream_t)stream, nullptr, ws,
+                      need, do_CC != 0, use_cache ? &cache : nullptr, force, part);
 }
 
 int rc_scratch_release(void *stream) { return scratch_release((hipStream_t)stream); }

@@ -102,6 +102,103 @@ hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const
     return hipGetLastError();
 }
 
+// Single unfiltered image of the drop-in: the same outputs as prep_kernel for
+// the image's pixels reordered so that those a rotation can bring inside the
+// projector's mask come first (index order), n_in of them; the others are
+// outside the mask for every rotation (their projections are 0, so they add
+// corr/2 |X|^2 to the score and nothing else) and enter xnorm only.  A pixel
+// is kept when (float)(x^2 + y^2) * pad2 <= thr, thr = (maxR2_padded + 1)
+// (1 + 2^-12): int(r^2) <= maxR2_padded needs r^2 < maxR2_padded + 1, and the
+// rotated float32 coordinates change r^2 by far less than 2^-12 of it.  The
+// host counts the same float expression to size the search.  One workgroup:
+// each round of 1024 pixels is compacted by wave ballots and a scan of the
+// 16 waves' counts.
+constexpr int kPartT = 1024;
+
+__global__ __launch_bounds__(kPartT) void prep_partition_kernel(
+    int image_size, int kpad, const float *__restrict__ real, const float *__restrict__ imag,
+    const float *__restrict__ corr, int imgX, int imgY, int maxR, float pad2, float thr,
+    float2 *__restrict__ xw, float2 *__restrict__ coords, double *__restrict__ xnorm,
+    float *__restrict__ half_corr, float *__restrict__ ascale, int cc_coords)
+{
+    __shared__ int s_cnt[kPartT / 64];
+    __shared__ double s_acc[kPartT];
+    __shared__ float s_max[kPartT];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double acc = 0.0;
+    float amax = 0.f;
+    int base = 0;
+    for (int r0 = 0; r0 < image_size; r0 += kPartT) {
+        const int p = r0 + tid;
+        bool in = false;
+        float2 w = make_float2(0.f, 0.f), cxy = make_float2(0.f, 0.f);
+        float hc = 0.f;
+        if (p < image_size) {
+            int x = p % imgX, y = p / imgX;
+            if (y > maxR) {
+                if (!cc_coords || y >= imgY - maxR)
+                    y -= imgY;
+                else
+                    x = maxR;
+            }
+            in = (float)(x * x + y * y) * pad2 <= thr;
+            const float xr = real[p], xi = imag[p], c = corr[p];
+            hc = c * 0.5f;
+            acc += (double)hc * ((double)xr * xr + (double)xi * xi);
+            w = make_float2(c * xr, c * xi);
+            cxy = make_float2((float)x, (float)y);
+        }
+        const unsigned long long m = __ballot(in);
+        if (lane == 0) s_cnt[wave] = __popcll(m);
+        __syncthreads();
+        int before = 0, total = 0;
+        for (int j = 0; j < kPartT / 64; ++j) {
+            const int c = s_cnt[j];
+            before += j < wave ? c : 0;
+            total += c;
+        }
+        if (in) {
+            const int dst = base + before + __popcll(m & ((1ull << lane) - 1ull));
+            xw[dst] = w;
+            coords[dst] = cxy;
+            half_corr[dst] = hc;
+            amax = fmaxf(amax, fmaxf(fabsf(w.x), fabsf(w.y)));
+        }
+        base += total;
+        __syncthreads();   // s_cnt is rewritten by the next round
+    }
+    for (int p = base + tid; p < kpad; p += kPartT) {
+        xw[p] = make_float2(0.f, 0.f);
+        coords[p] = make_float2(0.f, 0.f);
+        half_corr[p] = 0.f;
+    }
+    s_acc[tid] = acc;
+    s_max[tid] = amax;
+    __syncthreads();
+    for (int w = kPartT / 2; w > 0; w >>= 1) {
+        if (tid < w) {
+            s_acc[tid] += s_acc[tid + w];
+            s_max[tid] = fmaxf(s_max[tid], s_max[tid + w]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        xnorm[0] = s_acc[0];
+        if (ascale) ascale[0] = pow2_scale(s_max[0]);
+    }
+}
+
+hipError_t launch_prep_partition(int image_size, int kpad, const float *real, const float *imag,
+                                 const float *corr, int imgX, int imgY, int maxR, float pad2,
+                                 float thr, float2 *xw, float2 *coords, double *xnorm,
+                                 float *half_corr, float *ascale, hipStream_t s, bool cc_coords)
+{
+    hipLaunchKernelGGL(prep_partition_kernel, dim3(1), dim3(kPartT), 0, s, image_size, kpad, real,
+                       imag, corr, imgX, imgY, maxR, pad2, thr, xw, coords, xnorm, half_corr,
+                       ascale, (int)cc_coords);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // projection precompute: block = 8 orientations x one 32-pixel chunk
 // ---------------------------------------------------------------------------

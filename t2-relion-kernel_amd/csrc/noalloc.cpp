@@ -58,16 +58,20 @@ extern "C" int rc_run_diff2_kernel_coarse_noalloc(
     if (int st = check_drop_in(projector, orientation_num, translation_num, image_size,
                                data_is_3D))
         return st;
-    const size_t need = single_workspace_bytes((int)translation_num, (int)orientation_num,
-                                               (int)image_size, RC_FORM_FP32);
+    float pad2, thr;
+    const int n_in = dropin_pixels(*projector, (int)image_size, do_CC != 0, &pad2, &thr);
+    const bool part = n_in > 0;
+    const int npix = part ? n_in : (int)image_size;
+    const size_t need = single_workspace_bytes((int)translation_num, (int)orientation_num, npix,
+                                               RC_FORM_FP32);
     RC_REQUIRE(need <= kScratchBytes, RC_ERR_LIMIT,
                "this shape needs %zu bytes of scratch, beyond the %zu-byte static scratch of "
                "the allocation-free entry point (use rc_run_diff2_kernel_coarse)",
                need, kScratchBytes);
     void *ws = nullptr;
     if (int st = scratch_ptr(&ws)) return st;
-    return run_single_ws((int)translation_num, (int)orientation_num, (int)image_size,
-                         (int)image_size, RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y,
-                         d_fimg_real, d_fimg_imag, projector, d_corr_img, d_diff2s, d_diff2s,
-                         (hipStream_t)stream, nullptr, ws, need, do_CC != 0);
+    return run_single_ws((int)translation_num, (int)orientation_num, npix, (int)image_size,
+                         RC_FORM_FP32, d_eulers, d_trans_x, d_trans_y, d_fimg_real, d_fimg_imag,
+                         projector, d_corr_img, d_diff2s, d_diff2s, (hipStream_t)stream, nullptr,
+                         ws, need, do_CC != 0, part);
 }

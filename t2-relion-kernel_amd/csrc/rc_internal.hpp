@@ -162,6 +162,13 @@ hipError_t launch_prep(int n_particles, int npix, int full_npix, int kpad, const
                        const float *imag, const float *corr, const float *coor_xy, int imgX,
                        int imgY, int maxR, float2 *xw, float2 *coords, double *xnorm,
                        float *half_corr, float *ascale, hipStream_t s, bool cc_coords = false);
+// prep of one unfiltered drop-in image with the pixels a rotation can bring
+// inside the mask moved first: (float)(x^2 + y^2) * pad2 <= thr (see
+// dropin_pixels, which counts them on the host)
+hipError_t launch_prep_partition(int image_size, int kpad, const float *real, const float *imag,
+                                 const float *corr, int imgX, int imgY, int maxR, float pad2,
+                                 float thr, float2 *xw, float2 *coords, double *xnorm,
+                                 float *half_corr, float *ascale, hipStream_t s, bool cc_coords);
 
 // Projections of every (class, orientation, pixel) into the encoded layout.
 hipError_t launch_project(int form, const ProjView *vols, const float *bscale, int n_classes,
@@ -219,7 +226,14 @@ int single_split(int T, int O, int npix, int form, int *ksplit, int *kc_per_spli
 int run_single_ws(int T, int O, int npix, int full_npix, int form, const float *eulers,
                   const float *trans_x, const float *trans_y, const float *real, const float *imag,
                   const rc_projector_kernel *pk, const float *corr, const float *init, float *dest,
-                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc);
+                  hipStream_t stream, const float *coor_xy, void *ws, size_t ws_bytes, bool cc,
+                  bool partition);
+// Pixels of an unfiltered drop-in image of image_size pixels that a rotation
+// can bring inside the projector's mask (the search runs over these, moved
+// first by launch_prep_partition); 0 when none is.  *pad2 and *thr receive
+// the test's float constants.
+int dropin_pixels(const rc_projector_kernel &k, int image_size, bool cc, float *pad2,
+                  float *thr);
 // Argument checks of rc_run_diff2_kernel_coarse.
 int check_drop_in(const rc_projector_kernel *pk, unsigned long O, unsigned long T,
                   unsigned long image_size, int data_is_3D);

@@ -57,19 +57,32 @@ def test_noalloc_matches_allocating_entry_point(torch_cuda):
 
 
 def test_noalloc_rejects_oversized_shape(torch_cuda):
-    """113 translations x 4608 orientations x 1,000,000 pixels (within every
-    other limit) needs 16.8 MB of split partials plus 20 MB of image staging,
-    beyond the 32 MiB static scratch: RC_ERR_LIMIT before any launch."""
+    """64 translations x 61440 orientations over a 708 x 1414 half-plane image
+    whose mask (radius 707) holds ~786,000 pixels: ~31.5 MB of split partials
+    plus ~15.7 MB of image staging, beyond the 32 MiB static scratch:
+    RC_ERR_LIMIT before any launch (the device pointers are NULL)."""
+    import dataclasses
+
     from relion_amd import lib, synth
     L = lib.load()
     f = _noalloc(L)
-    need = L.rc_coarse_workspace_bytes(113, 4608, 1000000, 1000000, 0)
-    assert need > 32 << 20, need
     prob = synth.make_problem("mini", particles=1)
-    proj = lib.Projector(prob.vols[0])
-    pk = proj.kernel(1025, 1024, 512)
-    st = f(C.byref(pk), None, None, None, None, None, None, None, None, 1.0, 4608, 113,
-           1000000, None, 0, 0)
+    vol = dataclasses.replace(prob.vols[0], maxR=707)
+    proj = lib.Projector(vol)
+    imgX, imgY, maxR = 708, 1414, 707
+    pk = proj.kernel(imgX, imgY, maxR)
+    n = imgX * imgY
+    # the drop-in's pixel count: (float)(x^2 + y^2) * pad^2 <= (maxR2_padded + 1)(1 + 2^-12)
+    p = np.arange(n)
+    x, y = p % imgX, p // imgX
+    y = np.where(y > maxR, y - imgY, y)
+    pad2 = np.float32(pk.padding_factor) * np.float32(pk.padding_factor)
+    thr = (np.float32(pk.maxR2_padded) + np.float32(1)) * np.float32(1 + 2.0 ** -12)
+    n_in = int(np.count_nonzero((x * x + y * y).astype(np.float32) * pad2 <= thr))
+    need = L.rc_coarse_workspace_bytes(64, 61440, n_in, n_in, 0)
+    assert n_in > 700000 and need > 32 << 20, (n_in, need)
+    st = f(C.byref(pk), None, None, None, None, None, None, None, None, 1.0, 61440, 64, n, None,
+           0, 0)
     assert st == 5, st
     assert b"static scratch" in L.rc_last_error()
     proj.close()
