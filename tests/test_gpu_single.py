@@ -279,3 +279,62 @@ def test_autotune_choice_reaches_later_plans(lib, torch_cuda):
     sub = synth.make_problem("c1", seed=11, particles=3)
     other = lib.plan_for_problem(sub, projs, "bf16x3")
     assert other.tile()[0] in [x[0] for x in lib.variants("bf16x3")]
+
+
+@pytest.mark.parametrize("cc", [False, True])
+def test_drop_in_non_integer_padding(lib, torch_cuda, cc):
+    """The drop-in at padding factor 1.7 (maxR2_padded = int(16^2 1.7^2) =
+    739) against the float64 oracle over the unfiltered 32^2 image: pixels
+    with (x^2 + y^2) pad^2 just above 739 (739.84 at radius 16) still project
+    inside the mask (int(r^2) <= 739), so a search restricted by the filter
+    rule alone would drop them; the drop-in's restriction keeps every pixel a
+    rotation can bring inside."""
+    import dataclasses
+
+    from relion_amd import synth
+    torch = torch_cuda
+    prob = synth.make_problem("c1", particles=1, seed=11)
+    pb, g = prob.particles, prob.geom
+    inv = _inverse(synth.filter_order(g)[0])
+    vol = dataclasses.replace(prob.vols[0], pad=1.7)
+    proj = lib.Projector(vol)
+    pk = proj.kernel(g.imgX, g.imgY, g.maxR)
+    assert pk.maxR2_padded == 739
+    O = prob.orientation_num
+    tx, ty = prob.trans_of(0)
+    T = tx.size
+    re, im, corr = pb.real[0][inv], pb.imag[0][inv], pb.corr[0][inv]
+    p = np.arange(g.image_size)
+    x, y = p % g.imgX, p // g.imgX
+    if cc:
+        park = (y > g.maxR) & (y < g.imgY - g.maxR)
+        y = np.where((y > g.maxR) & ~park, y - g.imgY, y)
+        x = np.where(park, g.maxR, x)
+        ref = oracle.cc_f64(vol, pk.maxR2_padded, g.imgX, g.imgY, g.maxR, prob.eulers, tx, ty,
+                            re, im, corr).ravel()
+    else:
+        y = np.where(y > g.maxR, y - g.imgY, y)
+        ref = oracle.diff2_f64(vol, pk.maxR2_padded, prob.eulers, tx, ty, re, im, corr,
+                               x.astype(np.float64), y.astype(np.float64),
+                               n_proj=g.image_size).ravel()
+    # pixels beyond the filter rule that the mask still admits for some rotation
+    r2 = (x * x + y * y).astype(np.float32) * np.float32(pk.padding_factor) ** 2
+    assert np.count_nonzero((r2 > 739) & (r2 < 740)) > 0
+    d2 = torch.zeros(O * T, dtype=torch.float32, device="cuda")
+    lib.run_diff2_kernel_coarse(pk, _t(torch, tx), _t(torch, ty), _t(torch, corr),
+                                _t(torch, re), _t(torch, im), _t(torch, prob.eulers.ravel()), d2,
+                                O, T, g.image_size, do_CC=cc)
+    torch.cuda.synchronize()
+    got = d2.cpu().numpy()
+    if cc:
+        # the score crosses zero: its error against the score's scale, with
+        # the bound of the 32^2 drop-in CC test
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        print(f"drop-in pad 1.7 CC: max abs err / max |score| = {err:.3e}")
+        assert err <= 1e-5
+    else:
+        # the bounds of the FP32 form on the 32^2 workload
+        st = oracle.error_stats(got, ref)
+        print(f"drop-in pad 1.7: max_rel={st['max_rel']:.3e} mean_rel={st['mean_rel']:.3e}")
+        assert st["max_rel"] <= 1.2e-6 and st["mean_rel"] <= 2.2e-7, st
+    proj.close()
