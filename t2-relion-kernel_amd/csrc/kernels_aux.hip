@@ -157,8 +157,10 @@ __global__ __launch_bounds__(kPartT) void prep_partition_kernel(
             before += j < wave ? c : 0;
             total += c;
         }
-        if (in) {
-            const int dst = base + before + __popcll(m & ((1ull << lane) - 1ull));
+        // (dst < kpad always holds: the host sized kpad by the same count;
+        // the test keeps a disagreement from writing past the buffers)
+        const int dst = base + before + __popcll(m & ((1ull << lane) - 1ull));
+        if (in && dst < kpad) {
             xw[dst] = w;
             coords[dst] = cxy;
             half_corr[dst] = hc;
