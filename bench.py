@@ -57,6 +57,31 @@ def time_form(lib, torch, prob, projs, form, steps, warmup, tune=True):
     return res, t, plan
 
 
+def graph_replay(torch, plan, t, steps, warmup):
+    """Captures one step of the plan into a hipGraph, replays it warmup +
+    steps times on a side stream and times the steps with events on that
+    stream; the last replay's per-particle minima, arg-min and float32 diff2
+    checksum are compared with the eager run's (already in t)."""
+    best0 = t["best_index"].clone()
+    min0 = t["best_diff2"].clone()
+    sum0 = float(t["diff2"].sum())
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.capture(t, s)
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        plan.replay(s)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        plan.replay(s)
+    e1.record(s)
+    torch.cuda.synchronize()
+    same = (bool(torch.equal(t["best_index"], best0)) and bool(torch.equal(t["best_diff2"], min0))
+            and float(t["diff2"].sum()) == sum0)
+    return e0.elapsed_time(e1) / steps, same
+
+
 def measured_traffic(kernel, form, prob, npix):
     """HBM bytes per launch of the timed kernel on this workload, from the
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary kept in profiles/ (counters
@@ -260,6 +285,11 @@ def c4_bench(lib, torch, steps, warmup, tune):
                          frac_of_peak=flops / (cross_ms * 1e-3) / 1e12 / peak,
                          variant=plan.tile(), kernel=lib.variant_name(form, plan.tile()[0]),
                          autotuned=tuned is not None)
+        if form == "bf16x3":
+            # the whole batch as one captured hipGraph launch, replayed
+            gms, same = graph_replay(torch, plan, t, max(steps // 2, 1), 1)
+            out[form].update(graph_ms_per_step=gms, graph_evals_per_s=evals / (gms * 1e-3),
+                             graph_matches_eager=same)
         plan.close()
         log(f"c4 {form}: {out[form]}")
     del t
@@ -499,11 +529,17 @@ def main():
         r["evals_per_s"] = evals / (r["ms_per_step"] * 1e-3)
         flops = 4.0 * npix * evals
         r["cross_tflops"] = flops / (r["cross_ms"] * 1e-3) / 1e12
+        # the same step captured once into a hipGraph and replayed (it must
+        # reproduce the eager output, which the float64 check below reads)
+        gms, same = graph_replay(torch, plan, t, args.steps, min(args.warmup, 5))
+        r.update(graph_ms_per_step=gms, graph_evals_per_s=evals / (gms * 1e-3),
+                 graph_matches_eager=same)
         results[form] = r
         outs[form] = t
         plan.close()
         log(f"{form}: {r['ms_per_step']:.3f} ms/step, cross {r['cross_ms']:.3f} ms, "
-            f"{r['evals_per_s']:.3e} evals/s, {r['cross_tflops']:.1f} TFLOP/s, tile {r['variant']}")
+            f"{r['evals_per_s']:.3e} evals/s, {r['cross_tflops']:.1f} TFLOP/s, tile {r['variant']}; "
+            f"hipGraph replay {gms:.3f} ms/step (matches eager: {same})")
 
     n_chk = max(1, min(args.check_particles, prob.n_particles))
     picks = sorted(set(np.linspace(0, prob.n_particles - 1, n_chk).astype(int).tolist()))
