@@ -141,3 +141,16 @@ def test_new_entry_points_validate_before_launch():
     st = L.rc_run_diff2_kernel_coarse_noalloc(C.byref(k), None, None, None, None, None, None,
                                               None, None, 1.0, 1, 1, 1, None, 0, 0)
     assert st == 1
+
+
+def test_c_caller_example_is_built():
+    """The C-ABI-only example caller is built with the library and finds it
+    next to itself (rpath $ORIGIN/..)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "t2-relion-kernel_amd", "examples", "coarse_search")
+    assert os.access(exe, os.X_OK), exe
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    line = [ln for ln in out.splitlines() if "librelion_coarse_amd" in ln]
+    assert line and "not found" not in line[0], out
+
