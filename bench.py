@@ -223,11 +223,20 @@ def single_image_bench(lib, torch, cpu_seconds):
         call = lambda: lib.run_diff2_kernel_coarse(pk, *ins[:5], ins[5], diff2, O, T,
                                                    g.image_size)
         ms = event_ms(torch, call, 50)
-        ks, kc = lib.coarse_split(T, O, g.image_size)
+        # the pixels the drop-in searches: those a rotation can bring inside
+        # the projector's mask, by the library's float32 rule
+        # (x^2 + y^2) * pad^2 <= (maxR2_padded + 1) (1 + 2^-12)
+        p = np.arange(g.image_size)
+        x, y = p % g.imgX, p // g.imgX
+        y = np.where(y > g.maxR, y - g.imgY, y)
+        pad2 = np.float32(pk.padding_factor) * np.float32(pk.padding_factor)
+        thr = (np.float32(pk.maxR2_padded) + np.float32(1)) * np.float32(1 + 2.0 ** -12)
+        n_in = int(np.count_nonzero((x * x + y * y).astype(np.float32) * pad2 <= thr))
+        ks, kc = lib.coarse_split(T, O, n_in)
         rec = dict(workload=f"{T} translations x {O} orientations x {g.image_size} px "
                             f"(unfiltered, {pb.npix} in radius)",
-                   us_per_call=ms * 1e3, evals_per_s=O * T / (ms * 1e-3), split_slices=ks,
-                   chunks_per_slice=kc)
+                   us_per_call=ms * 1e3, evals_per_s=O * T / (ms * 1e-3), searched_px=n_in,
+                   split_slices=ks, chunks_per_slice=kc)
         if cpu_seconds > 0:
             from oracle import oracle
             t0 = time.perf_counter()

@@ -139,9 +139,9 @@ int rc_run_diff2_kernel_coarse(const rc_projector_kernel *projector,
  * reference; for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops).  Same arguments, semantics, kernels
  * and results (bit-identical) as rc_run_diff2_kernel_coarse; its scratch is
- * a 32 MiB device array inside the library (one per GPU, held until the
+ * a 24 MiB device array inside the library (one per GPU, held until the
  * process ends), so shapes whose workspace (rc_coarse_workspace_bytes with
- * the FP32 form) exceeds 32 MiB are rejected with RC_ERR_LIMIT before any
+ * the FP32 form) exceeds 24 MiB are rejected with RC_ERR_LIMIT before any
  * launch.  Calls that overlap on one GPU -- any two streams, or host threads
  * -- share that array and race: serialise them. */
 int rc_run_diff2_kernel_coarse_noalloc(const rc_projector_kernel *projector,
@@ -167,7 +167,7 @@ size_t rc_coarse_workspace_bytes(int translation_num, int orientation_num, int i
 
 /* Slices of the pixel axis the single-image calls use for a shape (the
  * counterpart of the split-K count of coarse_scheduler.cuh:211-213, here
- * min(chunks, ceil(512 / tiles)) bounded by 32 MiB of partials, with no empty
+ * min(chunks, max(1, floor(512 / tiles))) bounded by 32 MiB of partials, with no empty
  * slice): ksplit slices of kc_per_split 32-pixel chunks.  Diagnostic. */
 int rc_coarse_split(int translation_num, int orientation_num, int image_size, int form,
                     int *ksplit, int *kc_per_split);

@@ -90,7 +90,12 @@ static int check_kernel_view(const rc_projector_kernel *k)
 // to the workspace, and a second kernel adds the slices in a fixed order in
 // float64.  The slice count follows from the shape alone -- never from the
 // device or from timing -- so a shape always gives the same bits.
-constexpr int kSplitTargetBlocks = 512;            // workgroups the split aims for
+// The slice count is the largest that keeps the launch within
+// kSplitTargetBlocks workgroups -- two per CU of the 256 -- rather than the
+// smallest that reaches it: at one 113 x 4608 x 2112 particle (72 tiles) 7
+// slices (504 workgroups) take 76 us per call where 8 (576: a third
+// workgroup on some CUs) take 83, 14 (1008) 86 and 3 (216) 90.
+constexpr int kSplitTargetBlocks = 512;            // workgroups the split stays within
 constexpr size_t kSplitMaxBytes = 32ull << 20;     // cap of the cross-term partials
 
 struct SingleShape {
@@ -125,7 +130,7 @@ static SingleShape single_shape(int T, int O, int npix, int form)
     const long long tiles = ((rows + info->tile_m - 1) / info->tile_m) *
                             (((long long)O + info->tile_n - 1) / info->tile_n);
     sh.tiles = (int)std::min<long long>(tiles, 1 << 30);
-    long long want = (kSplitTargetBlocks + tiles - 1) / tiles;
+    long long want = kSplitTargetBlocks / tiles;
     const long long by_bytes = (long long)(kSplitMaxBytes / ((size_t)T * O * sizeof(float)));
     want = std::max(1ll, std::min({want, (long long)sh.nkc, by_bytes}));
     sh.kc_per_split = (int)((sh.nkc + want - 1) / want);

@@ -11,11 +11,11 @@
 
 namespace rc {
 
-// 32 MiB: image staging (20 bytes per pixel) plus the split-K partial sums,
-// which the slice rule caps at 32 MiB (a 113-translation, 4608-orientation,
-// 2112-pixel particle needs 16.9 MB in all).  Each GPU the library runs on
-// holds one copy until the process ends.
-constexpr size_t kScratchBytes = 32ull << 20;
+// 24 MiB: image staging (20 bytes per pixel) plus the split-K partial sums,
+// which the slice rule's 512-workgroup bound keeps under 17.1 MB (a
+// 113-translation, 4608-orientation, 2112-pixel particle needs 14.7 MB in
+// all).  Each GPU the library runs on holds one copy until the process ends.
+constexpr size_t kScratchBytes = 24ull << 20;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 // Device address of `scratch` on each device, resolved once per device with

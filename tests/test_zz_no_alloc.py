@@ -57,10 +57,11 @@ def test_noalloc_matches_allocating_entry_point(torch_cuda):
 
 
 def test_noalloc_rejects_oversized_shape(torch_cuda):
-    """64 translations x 61440 orientations over a 708 x 1414 half-plane image
-    whose mask (radius 707) holds ~786,000 pixels: ~31.5 MB of split partials
-    plus ~15.7 MB of image staging, beyond the 32 MiB static scratch:
-    RC_ERR_LIMIT before any launch (the device pointers are NULL)."""
+    """64 translations x 32768 orientations (256 tiles: two pixel slices) over
+    a 708 x 1414 half-plane image whose mask (radius 707) holds ~786,000
+    pixels: ~17 MB of split partials plus ~15.7 MB of image staging, beyond
+    the 24 MiB static scratch: RC_ERR_LIMIT before any launch (the device
+    pointers are NULL)."""
     import dataclasses
 
     from relion_amd import lib, synth
@@ -79,9 +80,10 @@ def test_noalloc_rejects_oversized_shape(torch_cuda):
     pad2 = np.float32(pk.padding_factor) * np.float32(pk.padding_factor)
     thr = (np.float32(pk.maxR2_padded) + np.float32(1)) * np.float32(1 + 2.0 ** -12)
     n_in = int(np.count_nonzero((x * x + y * y).astype(np.float32) * pad2 <= thr))
-    need = L.rc_coarse_workspace_bytes(64, 61440, n_in, n_in, 0)
-    assert n_in > 700000 and need > 32 << 20, (n_in, need)
-    st = f(C.byref(pk), None, None, None, None, None, None, None, None, 1.0, 61440, 64, n, None,
+    need = L.rc_coarse_workspace_bytes(64, 32768, n_in, n_in, 0)
+    assert n_in > 700000 and need > 24 << 20, (n_in, need)
+    assert lib.coarse_split(64, 32768, n_in)[0] == 2
+    st = f(C.byref(pk), None, None, None, None, None, None, None, None, 1.0, 32768, 64, n, None,
            0, 0)
     assert st == 5, st
     assert b"static scratch" in L.rc_last_error()
