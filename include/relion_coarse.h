@@ -396,7 +396,8 @@ int rc_server_destroy(rc_server *s);
  * x = maxR).  A job whose orientation or translations fall outside
  * orientation_num / translation_num is skipped.  d_diff2s must hold
  * d_job_idx[b] + d_job_num[b] entries for every job, as in the reference.
- * data_is_3D: RC_ERR_UNSUPPORTED. */
+ * todo_blocks above 2^24 - 8 (one 256-thread workgroup per job):
+ * RC_ERR_LIMIT.  data_is_3D: RC_ERR_UNSUPPORTED. */
 int rc_run_diff2_kernel_fine(const rc_projector_kernel *projector, const float *d_eulers,
                              const float *d_real, const float *d_imag, const float *d_trans_x,
                              const float *d_trans_y, const float *d_trans_z, const float *d_corr,

@@ -1191,7 +1191,8 @@ int rc_run_diff2_kernel_fine(const rc_projector_kernel *projector, const float *
                "image_size %lu exceeds imgX*imgY", image_size);
     RC_REQUIRE(orientation_num > 0 && translation_num > 0, RC_ERR_INVALID,
                "orientation_num and translation_num must be positive");
-    RC_REQUIRE(todo_blocks < (1ul << 31), RC_ERR_LIMIT, "too many jobs");
+    // one 256-thread workgroup per job: the grid's work-items stay below 2^32
+    RC_REQUIRE(todo_blocks <= (1ul << 24) - 8, RC_ERR_LIMIT, "too many jobs");
     if (todo_blocks == 0) return RC_OK;
     RC_REQUIRE(d_eulers && d_real && d_imag && d_trans_x && d_trans_y && d_corr && d_diff2s &&
                    d_rot_idx && d_trans_idx && d_job_idx && d_job_num,

@@ -22,10 +22,17 @@ constexpr int kFT = 256;       // threads per job
 constexpr int kFChunk = 16;    // translations per pass over the image
 constexpr double kInv2PiF = 0.15915494309189533577;
 
-__global__ __launch_bounds__(kFT) void fine_kernel(const FineArgs a)
+__global__ __launch_bounds__(kFT) void fine_kernel(const FineArgs a, unsigned n_jobs)
 {
     __shared__ double s_red[kFChunk][kFT / 64];
-    const int b = blockIdx.x;
+    // XCD-aware job order: block i runs on XCD i % 8, which takes the
+    // contiguous job range [x * per, (x + 1) * per), so consecutive jobs --
+    // neighbouring orientations, whose central sections share voxels --
+    // gather through one L2
+    const unsigned per = (n_jobs + 7) / 8;
+    const unsigned bj = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    if (bj >= n_jobs) return;
+    const int b = (int)bj;
     const int tid = threadIdx.x;
     const unsigned long j0 = a.job_idx[b];
     const unsigned long ix = a.rot_idx[j0];
@@ -97,7 +104,8 @@ __global__ __launch_bounds__(kFT) void fine_kernel(const FineArgs a)
 
 hipError_t launch_fine(const FineArgs &a, unsigned long n_jobs, hipStream_t s)
 {
-    hipLaunchKernelGGL(fine_kernel, dim3((unsigned)n_jobs), dim3(kFT), 0, s, a);
+    const unsigned per = (unsigned)((n_jobs + 7) / 8);
+    hipLaunchKernelGGL(fine_kernel, dim3(8 * per), dim3(kFT), 0, s, a, (unsigned)n_jobs);
     return hipGetLastError();
 }
 

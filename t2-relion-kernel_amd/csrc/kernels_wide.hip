@@ -38,7 +38,13 @@ namespace rc {
 // projection precompute in the wide encoding: block = 16 orientations x
 // kProjChunks 16-pixel chunks; each thread samples one orientation at one
 // pixel of every chunk, so its orientation rows are loaded once and the
-// gathers of its samples are in flight together
+// gathers of its samples are in flight together.  One-dimensional grid,
+// XCD-aware: block b runs on XCD b % 8, which takes whole 16-orientation
+// groups (b % 8, b % 8 + 8, ...), all their pixel chunks together, so the
+// voxels of a group's central sections -- shared by its in-plane (psi)
+// rotations -- are gathered through one L2 (142 -> 126 us for 4608
+// orientations x 1648 pixels, against spreading a group's chunks over all
+// eight XCDs).
 // ---------------------------------------------------------------------------
 constexpr int kProjChunks = 2;
 
@@ -46,7 +52,8 @@ template <int FORM, bool DIRECT>
 __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__restrict__ vols,
                                                            const float *__restrict__ bscale,
                                                            const float *__restrict__ eulers,
-                                                           int O, const float2 *__restrict__ coords,
+                                                           int O, int n_classes,
+                                                           const float2 *__restrict__ coords,
                                                            int npix, int kpad,
                                                            unsigned char *__restrict__ proj,
                                                            float *__restrict__ pabs2)
@@ -54,9 +61,16 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
     constexpr int KB = WideEnc<FORM>::kBytes;
     __shared__ __attribute__((aligned(16))) unsigned char s_out[kProjChunks][16 * KB];
     const int nkc = kpad / 16;
-    const int cls = blockIdx.z;
+    const int n_pairs = (nkc + kProjChunks - 1) / kProjChunks;   // blocks per group
+    const int n_og = (O + 15) / 16;                               // groups per class
+    const int jx = (int)(blockIdx.x >> 3);
+    const int G = (int)(blockIdx.x & 7) + 8 * (jx / n_pairs);    // (class, group)
+    if (G >= n_og * n_classes) return;
+    const int bx = jx % n_pairs;
+    const int cls = G / n_og;
+    const int by = G - cls * n_og;
     const int ol = threadIdx.x >> 4, pl = threadIdx.x & 15;
-    const int o = blockIdx.y * 16 + ol;
+    const int o = by * 16 + ol;
     const ProjView v = vols[cls];
     float2 val[kProjChunks];
     float e[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -71,7 +85,7 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
     }
 #pragma unroll
     for (int h = 0; h < kProjChunks; ++h) {
-        const int p = (blockIdx.x * kProjChunks + h) * 16 + pl;
+        const int p = (bx * kProjChunks + h) * 16 + pl;
         val[h] = make_float2(0.f, 0.f);
         if (o < O && p < npix) {
             const float2 xy = coords[p];
@@ -81,7 +95,7 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
     const float sc = (FORM == kFP16) ? bscale[cls] : 1.f;
 #pragma unroll
     for (int h = 0; h < kProjChunks; ++h) {
-        const int kc = blockIdx.x * kProjChunks + h;
+        const int kc = bx * kProjChunks + h;
         wide_store_value<FORM>(s_out[h] + ol * KB, pl, val[h].x, val[h].y, sc);
         if (o < O && kc < nkc)
             pabs2[((size_t)cls * O + o) * kpad + kc * 16 + pl] =
@@ -91,12 +105,12 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
     constexpr int units = 16 * KB / 16;
 #pragma unroll
     for (int h = 0; h < kProjChunks; ++h) {
-        const int kc = blockIdx.x * kProjChunks + h;
+        const int kc = bx * kProjChunks + h;
         if (kc >= nkc) break;
         if constexpr (DIRECT) {
             // the whole group, orientations past O included (zeros)
             unsigned char *grp =
-                proj + (((size_t)cls * nkc + kc) * gridDim.y + blockIdx.y) * (16 * KB);
+                proj + (((size_t)cls * nkc + kc) * n_og + by) * (16 * KB);
             for (int u = threadIdx.x; u < units; u += 256) {
                 const int oll = u / (KB / 16), seg = u % (KB / 16);
                 *reinterpret_cast<uint4 *>(grp + direct_seg_offset<FORM>(oll, seg)) =
@@ -104,7 +118,7 @@ __global__ __launch_bounds__(256) void project_wide_kernel(const ProjView *__res
             }
         } else {
             for (int u = threadIdx.x; u < units; u += 256) {
-                const int oo = blockIdx.y * 16 + u / (KB / 16);
+                const int oo = by * 16 + u / (KB / 16);
                 if (oo < O) {
                     const int seg = u % (KB / 16);
                     const uint4 d = reinterpret_cast<const uint4 *>(s_out[h])[u];
@@ -128,10 +142,12 @@ hipError_t launch_project_wide(int form, const ProjView *vols, const float *bsca
                                int kpad, unsigned char *proj, float *pabs2, bool direct,
                                hipStream_t s)
 {
-    dim3 grid((kpad / 16 + kProjChunks - 1) / kProjChunks, (O + 15) / 16, n_classes);
+    const long long groups = (long long)((O + 15) / 16) * n_classes;
+    const long long pairs = (kpad / 16 + kProjChunks - 1) / kProjChunks;
+    const dim3 grid((unsigned)(8 * ((groups + 7) / 8) * pairs));
 #define RC_PW(F, D)                                                                         \
     hipLaunchKernelGGL((project_wide_kernel<F, D>), grid, dim3(256), 0, s, vols, bscale, eulers, \
-                       O, coords, npix, kpad, proj, pabs2)
+                       O, n_classes, coords, npix, kpad, proj, pabs2)
     switch (form) {
     case kFP32: if (direct) RC_PW(kFP32, true); else RC_PW(kFP32, false); break;
     case kBF16X3: if (direct) RC_PW(kBF16X3, true); else RC_PW(kBF16X3, false); break;
