@@ -202,13 +202,18 @@ hipError_t launch_prep_partition(int image_size, int kpad, const float *real, co
 }
 
 // ---------------------------------------------------------------------------
-// projection precompute: block = 8 orientations x one 32-pixel chunk
+// projection precompute: block = 8 orientations x one 32-pixel chunk.
+// One-dimensional grid, XCD-aware as project_wide_kernel's: block b runs on
+// XCD b % 8, which takes whole 8-orientation groups (b % 8, b % 8 + 8, ...)
+// with all their chunks, so a group's central sections are gathered
+// through one L2.
 // ---------------------------------------------------------------------------
 // proj[(c*O + o)*nkc + kc] = encoded 32 samples; pabs2[(c*O+o)*kpad + p] = |P|^2
 template <int FORM>
 __global__ __launch_bounds__(256) void project_kernel(const ProjView *__restrict__ vols,
                                                       const float *__restrict__ bscale,
                                                       const float *__restrict__ eulers, int O,
+                                                      int n_classes,
                                                       const float2 *__restrict__ coords, int npix,
                                                       int kpad, unsigned char *__restrict__ proj,
                                                       float *__restrict__ pabs2)
@@ -217,11 +222,16 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjView *__restrict
     constexpr int GB = group_bytes(FORM);
     __shared__ __attribute__((aligned(16))) unsigned char s_out[8 * CB];
     const int nkc = kpad / kChunk;
-    const int kc = blockIdx.x;
-    const int cls = blockIdx.z;
+    const int n_og = (O + 7) / 8;   // 8-orientation groups per class
+    const int jx = (int)(blockIdx.x >> 3);
+    const int G = (int)(blockIdx.x & 7) + 8 * (jx / nkc);
+    if (G >= n_og * n_classes) return;
+    const int kc = jx % nkc;
+    const int cls = G / n_og;
+    const int by = G - cls * n_og;
     const int ol = threadIdx.x >> 5;
     const int pl = threadIdx.x & 31;
-    const int o = blockIdx.y * 8 + ol;
+    const int o = by * 8 + ol;
     const int p = kc * kChunk + pl;
     const ProjView v = vols[cls];
     float2 val = make_float2(0.f, 0.f);
@@ -236,7 +246,7 @@ __global__ __launch_bounds__(256) void project_kernel(const ProjView *__restrict
     __syncthreads();
     constexpr int units = 8 * CB / 16;
     for (int u = threadIdx.x; u < units; u += 256) {
-        const int oo = blockIdx.y * 8 + u / (CB / 16);
+        const int oo = by * 8 + u / (CB / 16);
         if (oo < O) {
             const int seg = u % (CB / 16);
             uint4 d = reinterpret_cast<const uint4 *>(s_out)[u];
@@ -249,19 +259,20 @@ hipError_t launch_project(int form, const ProjView *vols, const float *bscale, i
                           const float *eulers, int O, const float2 *coords, int npix, int kpad,
                           unsigned char *proj, float *pabs2, hipStream_t s)
 {
-    dim3 grid(kpad / kChunk, (O + 7) / 8, n_classes);
+    const long long groups = (long long)((O + 7) / 8) * n_classes;
+    const dim3 grid((unsigned)(8 * ((groups + 7) / 8) * (kpad / kChunk)));
     switch (form) {
     case kFP32:
         hipLaunchKernelGGL(project_kernel<kFP32>, grid, dim3(256), 0, s, vols, bscale, eulers, O,
-                           coords, npix, kpad, proj, pabs2);
+                           n_classes, coords, npix, kpad, proj, pabs2);
         break;
     case kBF16X3:
         hipLaunchKernelGGL(project_kernel<kBF16X3>, grid, dim3(256), 0, s, vols, bscale, eulers,
-                           O, coords, npix, kpad, proj, pabs2);
+                           O, n_classes, coords, npix, kpad, proj, pabs2);
         break;
     default:
         hipLaunchKernelGGL(project_kernel<kFP16>, grid, dim3(256), 0, s, vols, bscale, eulers, O,
-                           coords, npix, kpad, proj, pabs2);
+                           n_classes, coords, npix, kpad, proj, pabs2);
         break;
     }
     return hipGetLastError();
